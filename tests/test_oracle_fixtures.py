@@ -1,0 +1,252 @@
+"""The oracle against the reference's own known-answer tests, closed forms
+and brute force. CPU only."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.special
+
+
+def test_lut_fixture_and_closed_form(oracle, fixtures):
+    f = fixtures["lut_osf1_kw3"]
+    t = oracle.load_lut(f["osf"], f["kw"])
+    assert t.size == f["K"]
+    for i, v in f["exact"].items():
+        assert t[int(i)] == v
+    for i, v in f["one_sided_lt"].items():
+        assert v - t[int(i)] < f["eps"]
+    # closed form: KB(sqrt(i/(K-1))) with scipy's I0 in FP64
+    for osf, kw in [(1.0, 3), (2.0, 3), (1.5, 3), (1.25, 5), (2.0, 4)]:
+        t = oracle.load_lut(osf, kw)
+        K = t.size
+        beta = math.pi * math.sqrt((kw / osf * (osf - 0.5)) ** 2 - 0.8)
+        i = np.arange(1, K - 1)
+        r = np.sqrt(i / (K - 1))
+        ref = scipy.special.i0(beta * np.sqrt(1 - r * r)) / scipy.special.i0(beta)
+        assert np.max(np.abs(t[1:-1] - ref)) < 2e-6
+        assert t[0] == 1.0 and t[-1] == 0.0
+
+
+def test_lut_sizes(oracle):
+    assert oracle.kernel_size(2.0, 3) == 682
+    assert oracle.kernel_size(1.5, 3) == 909
+    assert oracle.kernel_size(1.0, 3) == 1365
+    assert oracle.kernel_size(2.0, 3, oracle.LOOKUP_LINEAR) == 45
+    assert oracle.kernel_size(1.5, 3, oracle.LOOKUP_LINEAR) == 60
+    assert abs(oracle.lib().o_beta(3, 2.0) - 6.486076) < 1e-5
+    assert abs(oracle.lib().o_beta(3, 1.5) - 5.619852) < 1e-5
+
+
+def test_sector_counts(oracle, fixtures):
+    for G, sw, expect in fixtures["sector_count_per_dim"]["cases"]:
+        assert math.ceil(np.float32(G) / np.float32(sw)) == expect
+
+
+def test_factory_3d(oracle, fixtures):
+    f = fixtures["factory_3d"]
+    p = oracle.precompute(np.array(f["coords_soa"], np.float32), f["img"], f["osf"], f["kw"],
+                          f["sw"])
+    gi = p["info"]
+    assert gi.grid[0] == f["grid_width"]
+    assert gi.sector_count == f["sector_count"]
+    assert list(p["sector_data_count"]) == f["sector_data_count"]
+    assert list(p["sector_centers"][:3]) == f["center0"]
+    assert list(p["data_indices"]) == f["data_indices"]
+    ts = p["traj_sorted"].ravel()
+    for i, v in f["sorted_coords_checks"].items():
+        assert abs(ts[int(i)] - v) < f["eps"]
+
+
+def test_factory_2d(oracle, fixtures):
+    f = fixtures["factory_2d"]
+    p = oracle.precompute(np.array(f["coords_soa"], np.float32), f["img"], f["osf"], f["kw"],
+                          f["sw"])
+    gi = p["info"]
+    assert gi.grid_count == f["grid_count"]
+    assert gi.sector_count == f["sector_count"]
+    assert list(p["sector_data_count"]) == f["sector_data_count"]
+    assert list(p["sector_centers"][:2]) == f["center0"]
+
+
+def _brute_sector(c, G, sw):
+    # E3 restated independently in numpy: float32 product, float64 add,
+    # round half away from zero, floor of the double quotient, clamp.
+    prod = np.float32(c) * np.float32(G)
+    v = float(prod) + 0.5 * float(np.float32(G))
+    r = math.floor(abs(v) + 0.5) * (1 if v >= 0 else -1)
+    s = math.floor(r / float(np.float32(sw)))
+    smax = math.ceil(float(np.float32(G) / np.float32(sw))) - 1
+    return min(max(s, 0), smax)
+
+
+@pytest.mark.parametrize("G,sw", [(24, 8), (145, 8), (192, 8), (512, 8), (97, 4), (40, 16)])
+def test_assign_sectors_brute_force(oracle, G, sw):
+    rng = np.random.default_rng(G * 100 + sw)
+    M = 2000
+    c = rng.uniform(-0.5, 0.5, size=M).astype(np.float32)
+    # include exact half-grid ties and the ends of the range
+    extra = (np.arange(-G // 2, G // 2 + 1) / G).astype(np.float32)
+    c = np.concatenate([c, extra, np.float32([-0.5, 0.5, 0.49999997, -0.49999997])])
+    traj = np.stack([c, np.zeros_like(c)])
+    gi = oracle.make_info((G, G), 1.0, 3, sw)
+    ids = oracle.assign_sectors(gi, traj)
+    sx = ids % gi.sectors[0]
+    expect = np.array([_brute_sector(v, G, sw) for v in c])
+    assert np.array_equal(sx, expect)
+
+
+def test_sort_std_is_a_sector_sort(oracle):
+    rng = np.random.default_rng(0)
+    sec = rng.integers(0, 50, size=5000).astype(np.uint32)
+    idx, ssec = oracle.sort_std(sec)
+    assert np.all(np.diff(ssec.astype(np.int64)) >= 0)
+    assert np.array_equal(sec[idx], ssec)
+    assert np.array_equal(np.sort(idx), np.arange(5000))
+    # same per-sector index sets as the canonical stable order
+    st = np.argsort(sec, kind="stable")
+    for s in range(50):
+        a = set(idx[ssec == s].tolist())
+        b = set(st[sec[st] == s].tolist())
+        assert a == b
+    # <= 16 elements: insertion sort, hence stable
+    small = np.array([5, 3, 5, 1, 3, 5], np.uint32)
+    i2, _ = oracle.sort_std(small)
+    assert list(i2) == list(np.argsort(small, kind="stable"))
+
+
+def test_processing_order(oracle):
+    sdc = np.array([0, 300, 300, 310, 900], np.uint32)  # counts 300, 0, 10, 590
+    po = oracle.processing_order(sdc)
+    assert po.tolist() == [[3, 0], [3, 256], [3, 512], [0, 0], [0, 256], [2, 0]]
+
+
+def test_cpu_gridding_fixtures(oracle, fixtures):
+    for case in fixtures["cpu_gridding"]:
+        kern = oracle.load_lut(1.0, case["kw"])
+        data = np.array([complex(*d) for d in case["data"]], np.complex64)
+        g = oracle.cpu_gridding(data, np.array(case["coords"], np.float32), kern,
+                                case["sectors"], np.array(case["centers"]).ravel(), case["sw"],
+                                case["kw"], case["width"])
+        for x, y, z, v, e in case["expect"]:
+            assert abs(g[z, y, x].real - v) < e, (case["src"], x, y, z)
+
+
+def test_spread_matches_cpu_gridding_inside_grid(oracle):
+    """The wrap-mode spread equals the reference CPU gridding (drop mode) for
+    samples whose footprint stays inside the grid."""
+    rng = np.random.default_rng(3)
+    M, N = 300, 16
+    traj = rng.uniform(-0.35, 0.35, size=(3, M)).astype(np.float32)
+    data = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    op = oracle.Operator(traj, (N, N, N), 3, 8, 1.0, deapo=False)
+    p = op.pre
+    g_wrap = op.adjoint(data[None], stop=oracle.CONVOLUTION)[0]
+    crds = np.ascontiguousarray(p["traj_sorted"].T)
+    g_drop = oracle.cpu_gridding(data[p["data_indices"]], crds, op.lut,
+                                 p["sector_data_count"], p["sector_centers"], 8, 3, N)
+    assert oracle.rel_l2(g_wrap, g_drop) < 1e-6
+
+
+def test_fftshift_fixtures(oracle, fixtures):
+    for case in fixtures["fftshift"]:
+        x, y, z = case["dims"]
+        a = np.arange(x * y * z).reshape(z, y, x)
+        s = np.fft.fftshift(a)
+        assert list(s.ravel()) == case["expect"]
+        assert list(np.fft.ifftshift(a).ravel()) == case["expect"]  # even dims
+
+
+def test_fft_stage_fixture(oracle):
+    # CUDA/test/gpu/gpuNUFFT_fft_tests.cpp:142-200 (osr 1, kw 3, sw 5, 10^3)
+    op = oracle.Operator(np.zeros((3, 1), np.float32), (10, 10, 10), 3, 5, 1.0)
+    r = op.adjoint(np.array([[1 + 1j]], np.complex64), stop=oracle.FFT)[0]
+    assert abs(r[5, 5, 5].real - 0.2172) < 1e-3
+    assert abs(r[5, 4, 5].real - 0.1975) < 1e-2
+    assert abs(r[5, 5, 4].real - 0.1975) < 1e-2
+    assert abs(r[5, 1, 0].real - 0.001619) < 1e-2
+
+
+def test_forward_fixture(oracle, fixtures):
+    f = fixtures["forward_2d_32_4"]
+    c = np.array(f["coords_flat"], np.float32).reshape(2, 3)  # SoA
+    img = np.full((f["coils"], 32, 32), 1 + 1j, np.complex64)
+    for lk, key in [(oracle.LOOKUP_NEAREST, "expect_nearest"),
+                    (oracle.LOOKUP_LINEAR, "expect_texture")]:
+        op = oracle.Operator(c, f["img"], f["kw"], f["sw"], f["osf"], lookup=lk)
+        y = op.forward(img)
+        for ch in range(f["coils"]):
+            assert abs(y[ch, 0].real - f[key]) < f["eps"]
+            assert abs(y[ch, 0] - y[0, 0]) < 1e-5
+    # the source read literally (sqrt(Ng) compensation) is off by sqrt(Ng/Ni)
+    op = oracle.Operator(c, f["img"], f["kw"], f["sw"], f["osf"], deapo_norm="grid")
+    y_grid = op.forward(img)[0, 0].real
+    assert abs(y_grid * math.sqrt(op.gi.grid_count / op.gi.img_count) - f["expect_nearest"]) < 1e-4
+
+
+def test_forward_sens_fixture(oracle, fixtures):
+    f = fixtures["forward_2d_32_8_sens"]
+    c = np.array(f["coords_flat"], np.float32).reshape(2, 3)
+    sens = np.stack([np.full((32, 32), (k + 1) * (1 + 1j), np.complex64)
+                     for k in range(f["coils"])])
+    for lk in (oracle.LOOKUP_NEAREST, oracle.LOOKUP_LINEAR):
+        op = oracle.Operator(c, f["img"], f["kw"], f["sw"], f["osf"], dens=np.ones(3, np.float32),
+                             sens=sens, lookup=lk)
+        y = op.forward(np.full((1, 32, 32), 1 + 1j, np.complex64))
+        for ch in range(1, f["coils"]):
+            assert abs((ch + 1) * y[0, 0] - y[ch, 0]) < f["eps"] * max(1, abs(y[ch, 0]))
+
+
+def test_deapo_fixtures(oracle, fixtures):
+    f = fixtures["deapo_2d_30_b1"]
+    C = f["coils"]
+    op = oracle.Operator(np.zeros((2, 1), np.float32), f["img"], f["kw"], f["sw"], f["osf"],
+                         dens=np.ones(1, np.float32),
+                         sens=np.full((C, 32, 32), complex(*f["sens"]), np.complex64))
+    r = op.adjoint(np.full((C, 1), complex(*f["data"]), np.complex64))
+    assert r.shape[0] == 1
+    assert abs(r[0, 0, 0].real - f["expect"]) < f["eps"]
+    f = fixtures["deapo_3d_32_2coils"]
+    op = oracle.Operator(np.zeros((3, 1), np.float32), f["img"], f["kw"], f["sw"], f["osf"])
+    r = op.adjoint(np.full((2, 1), 1 + 0j, np.complex64))
+    x, y, z = f["pixel"]
+    assert abs(r[0, z, y, x].real - f["expect"]) < f["eps"]
+    assert abs(r[0, z, y, x] - r[1, z, y, x]) < 1e-7
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_adjointness(oracle, dim):
+    """<A x, y> = <x, A^H y> (python/test_nufftOp.py:103-105) with dens and sens."""
+    rng = np.random.default_rng(7 + dim)
+    img = (24, 20) if dim == 2 else (12, 10, 8)
+    M = 400
+    traj = rng.uniform(-0.5, 0.5, size=(dim, M)).astype(np.float32)
+    dens = rng.uniform(0.1, 1.0, size=M).astype(np.float32)
+    shp = img[::-1]
+    sens = (rng.standard_normal((3,) + shp) + 1j * rng.standard_normal((3,) + shp)).astype(
+        np.complex64)
+    for lookup in (oracle.LOOKUP_NEAREST, oracle.LOOKUP_LINEAR):
+        op = oracle.Operator(traj, img, 3, 8, 2.0, dens=dens, sens=sens, lookup=lookup)
+        x = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(np.complex64)
+        y = (rng.standard_normal((3, M)) + 1j * rng.standard_normal((3, M))).astype(np.complex64)
+        Ax = op.forward(x[None])
+        AHy = op.adjoint(y)[0]
+        lhs = np.vdot(Ax.astype(np.complex128), y.astype(np.complex128))
+        rhs = np.vdot(x.astype(np.complex128), AHy.astype(np.complex128))
+        assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+@pytest.mark.parametrize("dim,img,osf,kw", [(2, (32, 32), 2.0, 3), (2, (30, 26), 2.0, 5),
+                                            (3, (12, 12, 12), 2.0, 3)])
+def test_ndft_error(oracle, dim, img, osf, kw):
+    """Gridding error against the FP64 direct NDFT (stated, and bounded)."""
+    rng = np.random.default_rng(11)
+    M = 500
+    traj = rng.uniform(-0.5, 0.5, size=(dim, M)).astype(np.float32)
+    op = oracle.Operator(traj, img, kw, 8, osf)
+    y = (rng.standard_normal((1, M)) + 1j * rng.standard_normal((1, M))).astype(np.complex64)
+    x = op.adjoint(y)
+    # the operator is normalised by 1/sqrt(Ni) end to end
+    ref = oracle.ndft_adjoint(op.gi, traj, y) * math.sqrt(op.gi.grid_count / op.gi.img_count)
+    err = oracle.rel_l2(x, ref)
+    assert err < (0.05 if kw == 3 else 0.01), err
