@@ -1,0 +1,852 @@
+// C ABI of libgpunufft_hip.so (declarations: include/gpunufft_hip.h).
+//
+// A plan is one operator: geometry, lookup table, deapodization array, the
+// device-resident sector tables of one trajectory, optional coil
+// sensitivities, a coil-batched work grid and rocFFT plans. All operator
+// state is per plan (kernel arguments, no process-global symbols), so plans
+// on different streams run concurrently.
+
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../../include/gpunufft_hip.h"
+#include "gnufft_internal.hpp"
+
+using namespace gnufft;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(GNUFFT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define ROCFFT_TRY(expr)                                                                   \
+  do {                                                                                     \
+    rocfft_status s_ = (expr);                                                             \
+    if (s_ != rocfft_status_success)                                                       \
+      return fail(GNUFFT_ERR_FFT, std::string(#expr) + " failed with rocfft status " +     \
+                                      std::to_string((int)s_));                            \
+  } while (0)
+
+std::once_flag g_rocfft_once;
+
+// ---------------------------------------------------------------------------
+// Kaiser-Bessel lookup table, float arithmetic as the reference
+// (CUDA/src/gpuNUFFT_utils.cpp:3-107, CUDA/inc/gpuNUFFT_utils.hpp:60-75).
+// ---------------------------------------------------------------------------
+float bessel_i0(float x) {
+  float ax = std::fabs(x);
+  float ans, y;
+  if (ax < 3.75f) {
+    y = x / 3.75f, y = y * y;
+    ans = 1.0f + y * (3.5156229f + y * (3.0899424f + y * (1.2067492f +
+                  y * (0.2659732f + y * (0.360768e-1f + y * 0.45813e-2f)))));
+  } else {
+    y = 3.75f / ax;
+    ans = (std::exp(ax) / std::sqrt(ax)) *
+          (0.39894228f + y * (0.1328592e-1f + y * (0.225319e-2f + y * (-0.157565e-2f +
+           y * (0.916281e-2f + y * (-0.2057706e-1f + y * (0.2635537e-1f +
+           y * (-0.1647633e-1f + y * 0.392377e-2f))))))));
+  }
+  return ans;
+}
+
+float kb_beta(unsigned kw, float osf) {
+  return (float)M_PI * std::sqrt(std::pow((float)kw / osf * (osf - 0.5f), 2.0f) - 0.8f);
+}
+
+float kb_value(float radius, unsigned kw, float osf) {
+  return bessel_i0(kb_beta(kw, osf) * std::sqrt(1.0f - std::pow(radius, 2.0f))) /
+         bessel_i0(kb_beta(kw, osf));
+}
+
+long lut_size(float osf, unsigned kw, int lookup) {
+  if (lookup == GNUFFT_LOOKUP_NEAREST) {
+    long kosf = (long)(std::floor(0.91f / (osf * 0.001f)));
+    return (long)(kosf * (float)kw * 0.5);
+  }
+  long kosf = (long)(std::floor(std::sqrt(0.37 / ((osf * osf) * 0.0001f))));
+  return (long)(kosf * (float)kw * 0.5);
+}
+
+std::vector<float> make_lut(long K, unsigned kw, float osf) {
+  std::vector<float> t((size_t)K, 0.0f);
+  for (long i = 1; i < K - 1; i++) {
+    if (kw == 1) {
+      t[i] = 1.0f;
+    } else {
+      double r = (double)std::sqrt(i / (double)(K - 1));
+      t[i] = kb_value((float)r, kw, osf);
+    }
+  }
+  t[0] = 1.0f;
+  t[K - 1] = 0.0f;
+  return t;
+}
+
+// Sector index along one axis, host twin of the device assignment
+// (CUDA/inc/precomp_utils.hpp:34-48).
+#pragma clang fp contract(off)
+int host_sector_of(float coord, int G, int sw) {
+  double v = (double)(coord * (float)G) + 0.5 * (double)(float)G;
+  int sector = (int)std::floor(std::round(v) / (double)(float)sw);
+  int smax = (int)std::ceil((float)G / (float)sw);
+  if (sector >= smax) sector = smax - 1;
+  if (sector < 0) sector = 0;
+  return sector;
+}
+#pragma clang fp contract(on)
+
+struct FftPlans {
+  rocfft_plan fwd = nullptr, inv = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* work = nullptr;
+  size_t work_bytes = 0;
+};
+
+}  // namespace
+
+struct gnufft_plan_s {
+  int device = 0;
+  Geom g{};
+  uint32_t img[3] = {0, 0, 0}, grid[3] = {0, 0, 0}, sectors[3] = {1, 1, 1};
+  uint32_t S = 0;
+  int lookup = 0;
+  unsigned kw = 3, sw = 8;
+  float osf = 2.0f;
+  std::vector<float> lut;
+  float* lut_d = nullptr;
+  float* deapo_d = nullptr;
+  std::vector<float> deapo_h;
+  // trajectory tables
+  uint32_t M = 0;
+  float* traj_sorted_d = nullptr;
+  float* dens_sorted_d = nullptr;
+  uint32_t* idx_d = nullptr;
+  uint32_t* assigned_d = nullptr;  // sector id per sample, caller order
+  uint32_t* sdc_d = nullptr;
+  uint32_t* centers_d = nullptr;
+  Chunk* chunks_d = nullptr;
+  uint32_t n_chunks = 0;
+  uint32_t chunk = 512;
+  // sensitivities
+  float2* sens_d = nullptr;
+  uint32_t n_sens = 0;
+  // work grid, capacity in coils
+  float2* grid_d = nullptr;
+  uint32_t grid_coils = 0;
+  float2* img_tmp_d = nullptr;
+  uint32_t img_tmp_coils = 0;
+  std::map<uint32_t, FftPlans> ffts;
+};
+
+namespace {
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void free_traj(gnufft_plan p) {
+  dfree(p->traj_sorted_d);
+  dfree(p->dens_sorted_d);
+  dfree(p->idx_d);
+  dfree(p->assigned_d);
+  dfree(p->sdc_d);
+  dfree(p->centers_d);
+  dfree(p->chunks_d);
+  p->traj_sorted_d = p->dens_sorted_d = nullptr;
+  p->idx_d = p->assigned_d = p->sdc_d = p->centers_d = nullptr;
+  p->chunks_d = nullptr;
+  p->M = 0;
+  p->n_chunks = 0;
+  p->g.M = 0;
+}
+
+void free_ffts(gnufft_plan p) {
+  for (auto& kv : p->ffts) {
+    if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
+    if (kv.second.inv) rocfft_plan_destroy(kv.second.inv);
+    if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
+    dfree(kv.second.work);
+  }
+  p->ffts.clear();
+}
+
+// Deapodization, following the reference's numeric procedure
+// (CUDA/src/gpuNUFFT_operator_factory.cpp:359-448): one unit sample at k=0
+// spread with sector width 8 and the operator's lookup mode, INVERSE shift,
+// unnormalised inverse DFT, INVERSE shift, crop, 1/|.|.
+// The spread of one sample is its tap tensor of float products
+// (wz*wy)*wx exactly as the spread kernel forms them; its DFT is evaluated
+// axis by axis in double (the phases are separable), which reproduces the
+// reference's values also where |K| is tiny and the rounding of the tap
+// products is amplified (osf near 1). deapo = sqrt(Ng/Ni)/|K(p)|: the
+// compensation the reference's observed outputs correspond to.
+int compute_deapo(gnufft_plan p, hipStream_t s) {
+  Geom d = p->g;  // same table, radius and anisotropy; sector width 8
+  d.sw = 8;
+  d.P = 8 + 2 * (int)std::floor((float)p->kw / 2.0f);
+  d.Pmax = d.P - 1;
+  d.off = (int)std::floor((float)d.P / 2.0f);
+  const int D = d.dim;
+  typedef std::complex<double> cd;
+  // per axis: tap weights and the phase matrix E[t][x]
+  std::vector<float> w[3];
+  std::vector<cd> E[3];
+  int T[3] = {1, 1, 1};
+  for (int a = 0; a < 3; a++) {
+    const int N = d.N[a];
+    if (a >= D) {
+      w[a].assign(1, 1.0f);
+      E[a].assign((size_t)N, cd(1.0, 0.0));
+      continue;
+    }
+    const int G = d.G[a];
+    const int sec = host_sector_of(0.0f, G, 8);
+    const int center = sec * 8 + 4;
+    const float u = map_kspace_to_grid(0.0f, G, center, d.off);
+    int lo, hi;
+    tap_range(u, d.Pmax, d.kradius, &lo, &hi);
+    std::vector<int> pos;
+    for (int i = lo; i <= hi; i++) {
+      float wt;
+      if (!axis_weight(d, p->lut.data(), a, i, center, 0.0f, &wt)) continue;
+      w[a].push_back(wt);
+      pos.push_back(wrap(wrap(center - d.off + i, G) - d.shift_inv[a], G));
+    }
+    T[a] = (int)w[a].size();
+    E[a].resize((size_t)T[a] * N);
+    for (int t = 0; t < T[a]; t++)
+      for (int x = 0; x < N; x++) {
+        const int m = wrap(x + d.crop[a] + d.shift_inv[a], G);
+        const double ph = 2.0 * M_PI * (double)(((long long)pos[t] * m) % G) / (double)G;
+        E[a][(size_t)t * N + x] = cd(std::cos(ph), std::sin(ph));
+      }
+  }
+  const int Nx = d.N[0], Ny = d.N[1], Nz = d.N[2];
+  // A[tz][ty][x] = sum_tx W[tz][ty][tx] Ex[tx][x]
+  std::vector<cd> A((size_t)T[2] * T[1] * Nx), B((size_t)T[2] * Ny * Nx);
+  for (int tz = 0; tz < T[2]; tz++)
+    for (int ty = 0; ty < T[1]; ty++)
+      for (int x = 0; x < Nx; x++) {
+        cd acc(0.0, 0.0);
+        for (int tx = 0; tx < T[0]; tx++) {
+          const float wt = D == 3 ? (w[2][tz] * w[1][ty]) * w[0][tx] : w[1][ty] * w[0][tx];
+          acc += (double)wt * E[0][(size_t)tx * Nx + x];
+        }
+        A[((size_t)tz * T[1] + ty) * Nx + x] = acc;
+      }
+  // B[tz][y][x] = sum_ty A[tz][ty][x] Ey[ty][y]
+  for (int tz = 0; tz < T[2]; tz++)
+    for (int y = 0; y < Ny; y++)
+      for (int x = 0; x < Nx; x++) {
+        cd acc(0.0, 0.0);
+        for (int ty = 0; ty < T[1]; ty++)
+          acc += A[((size_t)tz * T[1] + ty) * Nx + x] * E[1][(size_t)ty * Ny + y];
+        B[((size_t)tz * Ny + y) * Nx + x] = acc;
+      }
+  const double norm = std::sqrt((double)d.Ng / (double)d.Ni);
+  std::vector<float> deapo((size_t)d.Ni);
+  for (int z = 0; z < Nz; z++)
+    for (int y = 0; y < Ny; y++)
+      for (int x = 0; x < Nx; x++) {
+        cd acc(0.0, 0.0);
+        for (int tz = 0; tz < T[2]; tz++)
+          acc += B[((size_t)tz * Ny + y) * Nx + x] * E[2][(size_t)tz * Nz + z];
+        deapo[((size_t)z * Ny + y) * Nx + x] = (float)(norm / std::abs(acc));
+      }
+  HIP_TRY(hipMemcpyAsync(p->deapo_d, deapo.data(), deapo.size() * sizeof(float),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return GNUFFT_OK;
+}
+
+int ensure_grid(gnufft_plan p, uint32_t coils) {
+  if (p->grid_coils >= coils) return GNUFFT_OK;
+  dfree(p->grid_d);
+  p->grid_d = nullptr;
+  p->grid_coils = 0;
+  HIP_TRY(hipMalloc(&p->grid_d, (size_t)coils * p->g.Ng * sizeof(float2)));
+  p->grid_coils = coils;
+  return GNUFFT_OK;
+}
+
+// Coils processed together: all of them unless the grid would take more
+// than half of the free device memory (coil chunking,
+// CUDA/src/gpuNUFFT_operator.cpp:327-368).
+uint32_t coil_batch(gnufft_plan p, uint32_t n_coils) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 1;
+  size_t per = (size_t)p->g.Ng * sizeof(float2) * 2;  // grid + FFT work
+  size_t avail = fr / 2 + (size_t)p->grid_coils * p->g.Ng * sizeof(float2);
+  size_t cc = per ? avail / per : n_coils;
+  if (cc < 1) cc = 1;
+  return (uint32_t)std::min<size_t>(cc, n_coils);
+}
+
+int get_fft(gnufft_plan p, uint32_t batch, FftPlans** out) {
+  auto it = p->ffts.find(batch);
+  if (it != p->ffts.end()) {
+    *out = &it->second;
+    return GNUFFT_OK;
+  }
+  FftPlans f;
+  size_t lengths[3] = {(size_t)p->g.G[0], (size_t)p->g.G[1], (size_t)p->g.G[2]};
+  const size_t dims = (size_t)p->g.dim;
+  ROCFFT_TRY(rocfft_plan_create(&f.fwd, rocfft_placement_inplace,
+                                rocfft_transform_type_complex_forward, rocfft_precision_single,
+                                dims, lengths, batch, nullptr));
+  ROCFFT_TRY(rocfft_plan_create(&f.inv, rocfft_placement_inplace,
+                                rocfft_transform_type_complex_inverse, rocfft_precision_single,
+                                dims, lengths, batch, nullptr));
+  size_t w1 = 0, w2 = 0;
+  ROCFFT_TRY(rocfft_plan_get_work_buffer_size(f.fwd, &w1));
+  ROCFFT_TRY(rocfft_plan_get_work_buffer_size(f.inv, &w2));
+  f.work_bytes = std::max(w1, w2);
+  if (f.work_bytes) HIP_TRY(hipMalloc(&f.work, f.work_bytes));
+  ROCFFT_TRY(rocfft_execution_info_create(&f.info));
+  if (f.work_bytes) ROCFFT_TRY(rocfft_execution_info_set_work_buffer(f.info, f.work, f.work_bytes));
+  auto ins = p->ffts.emplace(batch, f);
+  *out = &ins.first->second;
+  return GNUFFT_OK;
+}
+
+int run_fft(gnufft_plan p, uint32_t batch, bool inverse, void* buf, hipStream_t s) {
+  FftPlans* f = nullptr;
+  int rc = get_fft(p, batch, &f);
+  if (rc) return rc;
+  ROCFFT_TRY(rocfft_execution_info_set_stream(f->info, s));
+  void* bufs[1] = {buf};
+  ROCFFT_TRY(rocfft_execute(inverse ? f->inv : f->fwd, bufs, nullptr, f->info));
+  return GNUFFT_OK;
+}
+
+int copy_in(void** dst, const void* src, size_t bytes, int location, hipStream_t s) {
+  HIP_TRY(hipMalloc(dst, bytes ? bytes : 4));
+  if (bytes)
+    HIP_TRY(hipMemcpyAsync(*dst, src, bytes,
+                           location == GNUFFT_DEVICE ? hipMemcpyDeviceToDevice
+                                                     : hipMemcpyHostToDevice,
+                           s));
+  return GNUFFT_OK;
+}
+
+// Gridding work list from the sector offsets.
+int build_chunks(gnufft_plan p, hipStream_t s) {
+  const uint32_t S = p->S;
+  uint32_t* off_d = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  HIP_TRY(hipMalloc(&off_d, 2 * (S + 1) * sizeof(uint32_t)));
+  uint32_t* cnt_d = off_d + (S + 1);
+  hipError_t e = count_chunks(p->sdc_d, S, p->chunk, cnt_d, off_d, nullptr, &temp_bytes, s);
+  if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 4);
+  if (e == hipSuccess) e = count_chunks(p->sdc_d, S, p->chunk, cnt_d, off_d, temp, &temp_bytes, s);
+  uint32_t total = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&total, off_d + S, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipMalloc(&p->chunks_d, (size_t)(total ? total : 1) * sizeof(Chunk));
+  if (e == hipSuccess) e = launch_fill_chunks(p->sdc_d, off_d, S, p->chunk, p->chunks_d, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(off_d);
+  dfree(temp);
+  if (e != hipSuccess) return fail(GNUFFT_ERR_DEVICE, std::string("chunk list: ") + hipGetErrorString(e));
+  p->n_chunks = total;
+  return GNUFFT_OK;
+}
+
+bool valid_stop(int stop) { return stop >= 0 && stop <= 3; }
+
+}  // namespace
+
+extern "C" {
+
+const char* gnufft_last_error(void) { return g_last_error.c_str(); }
+
+int gnufft_abi_version(void) { return GNUFFT_ABI_VERSION; }
+
+int gnufft_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "gnufft_malloc: null ptr");
+  HIP_TRY(hipMalloc(ptr, bytes ? bytes : 4));
+  return GNUFFT_OK;
+}
+
+int gnufft_free(void* ptr) {
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return GNUFFT_OK;
+}
+
+int gnufft_memcpy(void* dst, const void* src, size_t bytes, int dst_loc, int src_loc,
+                  void* stream) {
+  if (bytes == 0) return GNUFFT_OK;
+  if (!dst || !src) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "gnufft_memcpy: null pointer");
+  hipMemcpyKind k = dst_loc == GNUFFT_DEVICE
+                        ? (src_loc == GNUFFT_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice)
+                        : (src_loc == GNUFFT_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost);
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+  if (dst_loc == GNUFFT_HOST || src_loc == GNUFFT_HOST)
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return GNUFFT_OK;
+}
+
+int gnufft_zero(void* ptr, size_t bytes, void* stream) {
+  if (!ptr && bytes) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "gnufft_zero: null pointer");
+  HIP_TRY(launch_zero(ptr, bytes, (hipStream_t)stream));
+  return GNUFFT_OK;
+}
+
+int gnufft_stream_synchronize(void* stream) {
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return GNUFFT_OK;
+}
+
+int gnufft_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t kernel_width,
+                       uint32_t sector_width, float osf, int lookup, void* stream) {
+  if (!plan || !img_dims) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: null argument");
+  *plan = nullptr;
+  if (img_dims[0] == 0 || img_dims[1] == 0)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: image width and height must be > 0");
+  if (kernel_width < 1 || kernel_width > 7)
+    return fail(GNUFFT_ERR_UNSUPPORTED, "plan_create: kernel width must be in [1, 7]");
+  if (sector_width < 1) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: sector width 0");
+  if (!(osf >= 1.0f)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: osf must be >= 1");
+  if (lookup != GNUFFT_LOOKUP_NEAREST && lookup != GNUFFT_LOOKUP_LINEAR)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: unknown lookup mode");
+  hipStream_t s = (hipStream_t)stream;
+  std::call_once(g_rocfft_once, []() { rocfft_setup(); });
+
+  gnufft_plan p = new (std::nothrow) gnufft_plan_s();
+  if (!p) return fail(GNUFFT_ERR_OUT_OF_MEMORY, "plan_create: host allocation");
+  (void)hipGetDevice(&p->device);
+  p->kw = kernel_width;
+  p->sw = sector_width;
+  p->osf = osf;
+  p->lookup = lookup;
+  Geom& g = p->g;
+  g.dim = img_dims[2] == 0 ? 2 : 3;
+  // grid dims: (IndType)(N * osf), CUDA/inc/gpuNUFFT_types.hpp:109-117
+  for (int a = 0; a < 3; a++) {
+    p->img[a] = img_dims[a];
+    p->grid[a] = (uint32_t)(img_dims[a] * osf);
+  }
+  if (p->grid[0] == 0 || p->grid[1] == 0 || (g.dim == 3 && p->grid[2] == 0)) {
+    delete p;
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: empty grid");
+  }
+  const uint32_t max_grid = std::max(std::max(p->grid[0], p->grid[1]), p->grid[2]);
+  for (int a = 0; a < 3; a++) {
+    const bool used = a < g.dim;
+    g.G[a] = used ? (int)p->grid[a] : 1;
+    g.N[a] = used ? (int)p->img[a] : 1;
+    p->sectors[a] = used ? (uint32_t)std::ceil((float)p->grid[a] / (float)sector_width) : 1;
+    g.S[a] = (int)p->sectors[a];
+    g.shift_inv[a] = used ? g.G[a] / 2 : 0;
+    g.shift_fwd[a] = used ? (g.G[a] + 1) / 2 : 0;
+    g.crop[a] = used ? (int)(uint32_t)((float)p->img[a] * (osf - 1.0f) / 2.0f) : 0;
+    g.aniso[a] = (float)p->grid[a] / (float)max_grid;
+  }
+  p->S = p->sectors[0] * p->sectors[1] * p->sectors[2];
+  g.sw = (int)sector_width;
+  g.kw = (int)kernel_width;
+  g.P = (int)sector_width + 2 * (int)std::floor((float)kernel_width / 2.0f);
+  g.Pmax = g.P - 1;
+  g.off = (int)std::floor((float)g.P / 2.0f);
+  g.lookup = lookup;
+  const long K = lut_size(osf, kernel_width, lookup);
+  if (K < 2 || K > 100000) {
+    delete p;
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_create: lookup table size out of range");
+  }
+  g.K = (int)K;
+  // radius constants, CUDA/src/gpuNUFFT_operator.cpp:92-140
+  const double kernel_radius = (double)kernel_width / 2.0;
+  const double radius = kernel_radius / (double)max_grid;
+  const double rsq = radius * radius;
+  const double inv = 1.0 / rsq;
+  g.kradius = (float)kernel_radius;
+  g.rsq = (float)rsq;
+  g.rsq_inv = (float)inv;
+  g.dist_mult = (float)((double)(K - 1) * inv);
+  g.Ng = (long long)g.G[0] * g.G[1] * g.G[2];
+  g.Ni = (long long)g.N[0] * g.N[1] * g.N[2];
+  g.fft_scale = 1.0f / (float)std::sqrt((float)g.Ng);
+  g.M = 0;
+  p->chunk = g.dim == 3 ? 512 : 256;
+
+  p->lut = make_lut(K, kernel_width, osf);
+  int rc = copy_in((void**)&p->lut_d, p->lut.data(), p->lut.size() * sizeof(float), GNUFFT_HOST, s);
+  if (rc == GNUFFT_OK) {
+    hipError_t e = hipMalloc(&p->deapo_d, (size_t)g.Ni * sizeof(float));
+    if (e != hipSuccess) rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, "plan_create: deapo allocation");
+  }
+  if (rc == GNUFFT_OK) rc = compute_deapo(p, s);
+  if (rc != GNUFFT_OK) {
+    gnufft_plan_destroy(p);
+    return rc;
+  }
+  *plan = p;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_destroy(gnufft_plan p) {
+  if (!p) return GNUFFT_OK;
+  (void)hipDeviceSynchronize();
+  free_traj(p);
+  free_ffts(p);
+  dfree(p->lut_d);
+  dfree(p->deapo_d);
+  dfree(p->sens_d);
+  dfree(p->grid_d);
+  dfree(p->img_tmp_d);
+  delete p;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_get_info(gnufft_plan p, gnufft_plan_info* info) {
+  if (!p || !info) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "plan_get_info: null argument");
+  memset(info, 0, sizeof(*info));
+  info->dim = p->g.dim;
+  for (int a = 0; a < 3; a++) {
+    info->img[a] = p->img[a];
+    info->grid[a] = p->grid[a];
+    info->sectors[a] = p->sectors[a];
+  }
+  info->sector_count = p->S;
+  info->sector_width = p->sw;
+  info->kernel_width = p->kw;
+  info->kernel_count = (uint32_t)p->g.K;
+  info->sector_pad_width = (uint32_t)p->g.P;
+  info->sector_offset = (uint32_t)p->g.off;
+  info->n_samples = p->M;
+  info->n_sens_coils = p->n_sens;
+  info->n_chunks = p->n_chunks;
+  info->lookup = p->lookup;
+  info->has_dens = p->dens_sorted_d != nullptr;
+  info->osf = p->osf;
+  info->radius_sqr = p->g.rsq;
+  info->dist_multiplier = p->g.dist_mult;
+  info->grid_count = (uint64_t)p->g.Ng;
+  info->img_count = (uint64_t)p->g.Ni;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_set_trajectory(gnufft_plan p, const float* traj, uint32_t n_samples,
+                               const float* dens, int location, void* stream) {
+  if (!p || (!traj && n_samples)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_trajectory: null");
+  hipStream_t s = (hipStream_t)stream;
+  free_traj(p);
+  const uint32_t M = n_samples;
+  const int D = p->g.dim;
+  float* traj_in = nullptr;
+  float* dens_in = nullptr;
+  uint32_t *keys = nullptr, *vals = nullptr, *keys_sorted = nullptr;
+  void* temp = nullptr;
+  int rc = GNUFFT_OK;
+  hipError_t e = hipSuccess;
+  size_t temp_bytes = 0;
+  int key_bits = 1;
+  while ((1ull << key_bits) < (unsigned long long)p->S) key_bits++;
+
+  rc = copy_in((void**)&traj_in, traj, (size_t)D * M * sizeof(float), location, s);
+  if (rc == GNUFFT_OK && dens) rc = copy_in((void**)&dens_in, dens, (size_t)M * 4, location, s);
+  if (rc != GNUFFT_OK) goto done;
+  e = hipMalloc(&keys, (size_t)(M ? M : 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&vals, (size_t)(M ? M : 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&keys_sorted, (size_t)(M ? M : 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&p->idx_d, (size_t)(M ? M : 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&p->traj_sorted_d, (size_t)(M ? M : 1) * D * 4);
+  if (e == hipSuccess && dens) e = hipMalloc(&p->dens_sorted_d, (size_t)(M ? M : 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&p->sdc_d, (size_t)(p->S + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&p->centers_d, (size_t)p->S * D * 4);
+  if (e != hipSuccess) {
+    rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, std::string("set_trajectory: ") + hipGetErrorString(e));
+    goto done;
+  }
+  e = launch_assign_sectors(p->g, traj_in, M, keys, vals, s);
+  if (e == hipSuccess && M) e = sort_pairs(nullptr, &temp_bytes, keys, keys_sorted, vals, p->idx_d, M, key_bits, s);
+  if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 4);
+  if (e == hipSuccess && M) e = sort_pairs(temp, &temp_bytes, keys, keys_sorted, vals, p->idx_d, M, key_bits, s);
+  if (e == hipSuccess) e = launch_sector_data_count(keys_sorted, M, p->S, p->sdc_d, s);
+  if (e == hipSuccess) e = launch_sector_centers(p->g, p->centers_d, s);
+  if (e == hipSuccess) e = launch_gather_traj(traj_in, dens_in, p->idx_d, M, D, p->traj_sorted_d, p->dens_sorted_d, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    rc = fail(GNUFFT_ERR_DEVICE, std::string("set_trajectory: ") + hipGetErrorString(e));
+    goto done;
+  }
+  p->assigned_d = keys;
+  keys = nullptr;
+  p->M = M;
+  p->g.M = (int)M;
+  rc = build_chunks(p, s);
+done:
+  dfree(traj_in);
+  dfree(dens_in);
+  dfree(keys);
+  dfree(vals);
+  dfree(keys_sorted);
+  dfree(temp);
+  if (rc != GNUFFT_OK) free_traj(p);
+  return rc;
+}
+
+int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
+                                 const uint32_t* data_indices, const uint32_t* sector_data_count,
+                                 const uint32_t* sector_centers, const float* dens_sorted,
+                                 const float* deapo, uint32_t n_samples, int location,
+                                 void* stream) {
+  if (!p || !sector_data_count || !sector_centers || ((!traj_sorted || !data_indices) && n_samples))
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "load_precomputed: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  free_traj(p);
+  const uint32_t M = n_samples;
+  const int D = p->g.dim;
+  int rc = copy_in((void**)&p->traj_sorted_d, traj_sorted, (size_t)D * M * 4, location, s);
+  if (!rc) rc = copy_in((void**)&p->idx_d, data_indices, (size_t)M * 4, location, s);
+  if (!rc) rc = copy_in((void**)&p->sdc_d, sector_data_count, (size_t)(p->S + 1) * 4, location, s);
+  if (!rc) rc = copy_in((void**)&p->centers_d, sector_centers, (size_t)p->S * D * 4, location, s);
+  if (!rc && dens_sorted) rc = copy_in((void**)&p->dens_sorted_d, dens_sorted, (size_t)M * 4, location, s);
+  if (!rc && deapo)
+    HIP_TRY(hipMemcpyAsync(p->deapo_d, deapo, (size_t)p->g.Ni * 4,
+                           location == GNUFFT_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           s));
+  if (!rc) {
+    HIP_TRY(hipStreamSynchronize(s));
+    p->M = M;
+    p->g.M = (int)M;
+    rc = build_chunks(p, s);
+  }
+  if (rc) free_traj(p);
+  return rc;
+}
+
+int gnufft_plan_set_sens(gnufft_plan p, const void* sens, uint32_t n_coils, int location,
+                         void* stream) {
+  if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_sens: null plan");
+  dfree(p->sens_d);
+  p->sens_d = nullptr;
+  p->n_sens = 0;
+  if (!sens || n_coils == 0) return GNUFFT_OK;
+  int rc = copy_in((void**)&p->sens_d, sens, (size_t)n_coils * p->g.Ni * sizeof(float2), location,
+                   (hipStream_t)stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  p->n_sens = n_coils;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_get_tables(gnufft_plan p, uint32_t* data_indices, uint32_t* sector_data_count,
+                           uint32_t* sector_centers, float* traj_sorted, float* dens_sorted,
+                           float* deapo, float* lut) {
+  if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_tables: null plan");
+  const int D = p->g.dim;
+  if ((data_indices || sector_data_count || sector_centers || traj_sorted || dens_sorted) &&
+      !p->sdc_d)
+    return fail(GNUFFT_ERR_STATE, "get_tables: no trajectory set");
+  if (data_indices && p->M) HIP_TRY(hipMemcpy(data_indices, p->idx_d, (size_t)p->M * 4, hipMemcpyDeviceToHost));
+  if (sector_data_count) HIP_TRY(hipMemcpy(sector_data_count, p->sdc_d, (size_t)(p->S + 1) * 4, hipMemcpyDeviceToHost));
+  if (sector_centers) HIP_TRY(hipMemcpy(sector_centers, p->centers_d, (size_t)p->S * D * 4, hipMemcpyDeviceToHost));
+  if (traj_sorted && p->M) HIP_TRY(hipMemcpy(traj_sorted, p->traj_sorted_d, (size_t)p->M * D * 4, hipMemcpyDeviceToHost));
+  if (dens_sorted) {
+    if (!p->dens_sorted_d) return fail(GNUFFT_ERR_STATE, "get_tables: no density compensation");
+    HIP_TRY(hipMemcpy(dens_sorted, p->dens_sorted_d, (size_t)p->M * 4, hipMemcpyDeviceToHost));
+  }
+  if (deapo) HIP_TRY(hipMemcpy(deapo, p->deapo_d, (size_t)p->g.Ni * 4, hipMemcpyDeviceToHost));
+  if (lut) memcpy(lut, p->lut.data(), p->lut.size() * sizeof(float));
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_get_assigned_sectors(gnufft_plan p, uint32_t* sectors_host) {
+  if (!p || !sectors_host) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_assigned_sectors: null");
+  if (!p->assigned_d) return fail(GNUFFT_ERR_STATE, "get_assigned_sectors: tables were loaded, not computed");
+  if (p->M) HIP_TRY(hipMemcpy(sectors_host, p->assigned_d, (size_t)p->M * 4, hipMemcpyDeviceToHost));
+  return GNUFFT_OK;
+}
+
+int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uint32_t n_coils,
+                         void* stream) {
+  if (!p || !grid || (!data_sorted && p->M)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "spread_sorted: null");
+  if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "spread_sorted: no trajectory set");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(launch_zero(grid, (size_t)n_coils * p->g.Ng * sizeof(float2), s));
+  SpreadArgs a{};
+  a.data = (const float2*)data_sorted;
+  a.coil_stride = p->M;
+  a.crds = p->traj_sorted_d;
+  a.chunks = p->chunks_d;
+  a.n_chunks = p->n_chunks;
+  a.centers = p->centers_d;
+  a.lut = p->lut_d;
+  a.grid = (float2*)grid;
+  a.n_coils = (int)n_coils;
+  a.fold_shift = false;
+  HIP_TRY(launch_spread(p->g, a, s));
+  return GNUFFT_OK;
+}
+
+int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uint32_t n_coils,
+                         void* stream) {
+  if (!p || !grid || (!data_sorted && p->M)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "interp_sorted: null");
+  if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "interp_sorted: no trajectory set");
+  InterpArgs a{};
+  a.grid = (const float2*)grid;
+  a.crds = p->traj_sorted_d;
+  a.chunks = p->chunks_d;
+  a.n_chunks = p->n_chunks;
+  a.centers = p->centers_d;
+  a.lut = p->lut_d;
+  a.out = (float2*)data_sorted;
+  a.coil_stride = p->M;
+  a.scale = 1.0f;
+  a.n_coils = (int)n_coils;
+  HIP_TRY(launch_interp(p->g, a, (hipStream_t)stream));
+  return GNUFFT_OK;
+}
+
+int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils, int stop,
+                   void* stream) {
+  if (!p || !out || (!kdata && p->M)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint: null argument");
+  if (!valid_stop(stop)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint: unknown stop stage");
+  if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "adjoint: no trajectory set");
+  if (n_coils == 0) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint: zero coils");
+  const bool use_sens = p->n_sens > 0 && stop == GNUFFT_STOP_DEAPODIZATION;
+  if (p->n_sens > 0 && p->n_sens != n_coils)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint: coil count differs from sensitivity coils");
+  hipStream_t s = (hipStream_t)stream;
+  const Geom& g = p->g;
+  const bool grid_out = stop == GNUFFT_STOP_CONVOLUTION || stop == GNUFFT_STOP_DENSITY_ESTIMATION;
+  const uint32_t cc = grid_out ? n_coils : coil_batch(p, n_coils);
+  if (!grid_out) {
+    int rc = ensure_grid(p, cc);
+    if (rc) return rc;
+  }
+  for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
+    const uint32_t nc = std::min(cc, n_coils - c0);
+    float2* grid = grid_out ? (float2*)out + (size_t)c0 * g.Ng : p->grid_d;
+    HIP_TRY(launch_zero(grid, (size_t)nc * g.Ng * sizeof(float2), s));
+    SpreadArgs a{};
+    a.data = (const float2*)kdata + (size_t)c0 * p->M;
+    a.coil_stride = p->M;
+    a.gather = p->idx_d;
+    a.dens = p->dens_sorted_d;
+    a.crds = p->traj_sorted_d;
+    a.chunks = p->chunks_d;
+    a.n_chunks = p->n_chunks;
+    a.centers = p->centers_d;
+    a.lut = p->lut_d;
+    a.grid = grid;
+    a.n_coils = (int)nc;
+    a.fold_shift = !grid_out;
+    HIP_TRY(launch_spread(g, a, s));
+    if (grid_out) continue;
+    int rc = run_fft(p, nc, true, grid, s);
+    if (rc) return rc;
+    if (use_sens) {
+      HIP_TRY(launch_adjoint_post(g, grid, (int)nc, p->deapo_d, p->sens_d + (size_t)c0 * g.Ni,
+                                  (float2*)out, c0 > 0, stop, s));
+    } else {
+      HIP_TRY(launch_adjoint_post(g, grid, (int)nc, p->deapo_d, nullptr,
+                                  (float2*)out + (size_t)c0 * g.Ni, false, stop, s));
+    }
+  }
+  return GNUFFT_OK;
+}
+
+int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, int stop,
+                   void* stream) {
+  if (!p || !img || (!out && p->M)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "forward: null argument");
+  if (!valid_stop(stop)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "forward: unknown stop stage");
+  if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "forward: no trajectory set");
+  if (n_coils == 0) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "forward: zero coils");
+  const bool use_sens = p->n_sens > 0;
+  if (use_sens && p->n_sens != n_coils)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "forward: coil count differs from sensitivity coils");
+  hipStream_t s = (hipStream_t)stream;
+  const Geom& g = p->g;
+  if (stop == GNUFFT_STOP_DENSITY_ESTIMATION) {
+    // interpolation from the deapodized image used as the grid
+    // (gpuNUFFT_operator.cpp:1178-1187): defined only for grid == image size
+    if (g.Ng != g.Ni)
+      return fail(GNUFFT_ERR_UNSUPPORTED, "forward DENSITY_ESTIMATION needs grid == image size (osf 1)");
+    if (p->img_tmp_coils < n_coils) {
+      dfree(p->img_tmp_d);
+      p->img_tmp_d = nullptr;
+      p->img_tmp_coils = 0;
+      HIP_TRY(hipMalloc(&p->img_tmp_d, (size_t)n_coils * g.Ni * sizeof(float2)));
+      p->img_tmp_coils = n_coils;
+    }
+    HIP_TRY(launch_image_prep(g, (const float2*)img, !use_sens, p->sens_d, p->deapo_d,
+                              p->img_tmp_d, (int)n_coils, s));
+    InterpArgs a{};
+    a.grid = p->img_tmp_d;
+    a.crds = p->traj_sorted_d;
+    a.chunks = p->chunks_d;
+    a.n_chunks = p->n_chunks;
+    a.centers = p->centers_d;
+    a.lut = p->lut_d;
+    a.out = (float2*)out;
+    a.coil_stride = p->M;
+    a.scatter = p->idx_d;
+    a.scale = 1.0f;
+    a.n_coils = (int)n_coils;
+    HIP_TRY(launch_interp(g, a, s));
+    return GNUFFT_OK;
+  }
+  const uint32_t cc = coil_batch(p, n_coils);
+  int rc = ensure_grid(p, cc);
+  if (rc) return rc;
+  for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
+    const uint32_t nc = std::min(cc, n_coils - c0);
+    const float2* src = use_sens ? (const float2*)img : (const float2*)img + (size_t)c0 * g.Ni;
+    HIP_TRY(launch_forward_pre(g, src, !use_sens, use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr,
+                               p->deapo_d, p->grid_d, (int)nc, s));
+    rc = run_fft(p, nc, false, p->grid_d, s);
+    if (rc) return rc;
+    InterpArgs a{};
+    a.grid = p->grid_d;
+    a.crds = p->traj_sorted_d;
+    a.chunks = p->chunks_d;
+    a.n_chunks = p->n_chunks;
+    a.centers = p->centers_d;
+    a.lut = p->lut_d;
+    a.out = (float2*)out + (size_t)c0 * p->M;
+    a.coil_stride = p->M;
+    a.scatter = p->idx_d;
+    a.dens = p->dens_sorted_d;
+    a.scale = g.fft_scale;
+    a.n_coils = (int)nc;
+    a.fold_shift = true;
+    HIP_TRY(launch_interp(g, a, s));
+  }
+  return GNUFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+// Internal declarations shared by the HIP translation units of
+// libgpunufft_hip.so. Not part of the public ABI (include/gpunufft_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace gnufft {
+
+// Geometry and kernel constants of one operator, passed to kernels by value.
+// Field meanings follow GpuNUFFTInfo (CUDA/inc/gpuNUFFT_types.hpp:230-311,
+// derived in CUDA/src/gpuNUFFT_operator.cpp:53-146).
+struct Geom {
+  int dim;          // 2 or 3
+  int G[3];         // grid dims; G[2] = 1 in 2-D
+  int N[3];         // image dims; N[2] = 1 in 2-D
+  int S[3];         // sectors per dim; S[2] = 1 in 2-D
+  int sw;           // sector width
+  int P;            // padded sector width sw + 2*floor(kw/2)
+  int Pmax;         // P - 1
+  int off;          // floor(P/2)
+  int kw;           // kernel width
+  int K;            // lookup table entries
+  int lookup;       // GNUFFT_LOOKUP_*
+  float kradius;    // kw/2
+  float rsq;        // squared kernel radius relative to the largest grid dim
+  float rsq_inv;
+  float dist_mult;  // (K-1)/rsq
+  float aniso[3];   // G_a / max(G)
+  int shift_inv[3]; // floor(G/2): INVERSE fftshift offset
+  int shift_fwd[3]; // ceil(G/2): FORWARD fftshift offset
+  int crop[3];      // (int)(N*(osf-1)/2)
+  long long Ng;     // grid points per coil
+  long long Ni;     // image points per coil
+  int M;            // samples
+  float fft_scale;  // 1/sqrt(Ng)
+};
+
+// One gridding work item: samples [begin, end) of a sector.
+struct Chunk {
+  uint32_t sector;
+  uint32_t begin;
+  uint32_t end;
+  uint32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Exact per-sample index / weight arithmetic. Contraction is disabled so the
+// float rounding of every ceil/floor/round input matches the reference's
+// separate multiply and add (CUDA/inc/gpuNUFFT_utils.hpp:89-104, 194-209).
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+__device__ __host__ inline float map_kspace_to_grid(float pos, int G, int center, int off) {
+  return (pos * (float)G) + (0.5f * (float)G) - (float)center + (float)off;
+}
+
+__device__ __host__ inline float map_grid_to_kspace(int i, int G, int center, int off) {
+  return ((float)i + (float)center - (float)off) / (float)G - 0.5f;
+}
+
+__device__ __host__ inline void tap_range(float u, int Pmax, float radius, int* lo, int* hi) {
+  int mn = (int)ceilf(u - radius);
+  int mx = (int)floorf(u + radius);
+  if (mn < 0) mn = 0;
+  if (mx < 0) mx = 0;
+  if (mx >= Pmax) mx = Pmax;
+  if (mn >= (float)Pmax) mn = (int)((float)Pmax - 2.0f * radius);
+  if (mn < 0) mn = 0;  // only reachable when P < kw + 1 (reference reads out of bounds)
+  *lo = mn;
+  *hi = mx;
+}
+
+// Emulated clamp-addressed linear texture fetch of the 1-D table at the
+// normalised coordinate u: texel coordinate u*K-0.5, fractional weight with
+// 8 fractional bits (round to nearest).
+__device__ __host__ inline float lerp_fetch(const float* tab, int K, float u) {
+  float xb = u * (float)K - 0.5f;
+  float fl = floorf(xb);
+  int i0 = (int)fl;
+  float a = xb - fl;
+  a = rintf(a * 256.0f) * (1.0f / 256.0f);
+  int i1 = i0 + 1;
+  i0 = i0 < 0 ? 0 : (i0 > K - 1 ? K - 1 : i0);
+  i1 = i1 < 0 ? 0 : (i1 > K - 1 ? K - 1 : i1);
+  return (1.0f - a) * tab[i0] + a * tab[i1];
+}
+
+// Weight of one tap along one axis; returns false when the nearest-mode
+// radius test rejects the tap (atomic_gpuNUFFT_kernels.cu:343-368); the
+// linear mode has no radius test (texture_gpuNUFFT_kernels.cu:63-82).
+__device__ __host__ inline bool axis_weight(const Geom& g, const float* lut, int a, int i,
+                                            int center, float pos, float* w) {
+  float kp = map_grid_to_kspace(i, g.G[a], center, g.off);
+  float d = (kp - pos) * g.aniso[a];
+  d *= d;
+  if (g.lookup == 0) {
+    if (!(d < g.rsq)) return false;
+    int li = (int)roundf(d * g.dist_mult);
+    *w = lut[li < g.K - 1 ? li : g.K - 1];
+    return true;
+  }
+  *w = lerp_fetch(lut, g.K, d * g.rsq_inv);
+  return true;
+}
+
+#pragma clang fp contract(on)
+
+__device__ __host__ inline int wrap(int p, int G) {
+  p %= G;
+  return p < 0 ? p + G : p;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers implemented in the .hip translation units (all asynchronous on
+// `stream`; return hipError_t).
+// ---------------------------------------------------------------------------
+
+// precompute.hip
+hipError_t launch_assign_sectors(const Geom& g, const float* traj, uint32_t M, uint32_t* keys,
+                                 uint32_t* vals, hipStream_t s);
+hipError_t sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in,
+                      uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                      uint32_t M, int key_bits, hipStream_t s);
+hipError_t launch_sector_data_count(const uint32_t* sorted_keys, uint32_t M, uint32_t S,
+                                    uint32_t* sdc, hipStream_t s);
+hipError_t launch_sector_centers(const Geom& g, uint32_t* centers, hipStream_t s);
+hipError_t launch_gather_traj(const float* traj, const float* dens, const uint32_t* idx,
+                              uint32_t M, int dim, float* traj_sorted, float* dens_sorted,
+                              hipStream_t s);
+hipError_t count_chunks(const uint32_t* sdc, uint32_t S, uint32_t chunk, uint32_t* cnt,
+                        uint32_t* chunk_off, void* temp, size_t* temp_bytes, hipStream_t s);
+hipError_t launch_fill_chunks(const uint32_t* sdc, const uint32_t* chunk_off, uint32_t S,
+                              uint32_t chunk, Chunk* chunks, hipStream_t s);
+
+// gridding.hip
+struct SpreadArgs {
+  const float2* data;      // [coil][...] samples
+  long long coil_stride;   // elements between coils of `data`
+  const uint32_t* gather;  // NULL: data already in sector order
+  const float* dens;       // NULL: no density weighting (sorted order)
+  const float* crds;       // sorted SoA [dim][M]
+  const Chunk* chunks;
+  uint32_t n_chunks;
+  const uint32_t* centers; // [S][dim]
+  const float* lut;
+  float2* grid;            // [coil][Ng]
+  int n_coils;
+  bool fold_shift;         // store at INVERSE-fftshifted positions
+};
+hipError_t launch_spread(const Geom& g, const SpreadArgs& a, hipStream_t s);
+
+struct InterpArgs {
+  const float2* grid;      // [coil][Ng]
+  const float* crds;
+  const Chunk* chunks;
+  uint32_t n_chunks;
+  const uint32_t* centers;
+  const float* lut;
+  float2* out;             // [coil][...]
+  long long coil_stride;
+  const uint32_t* scatter; // NULL: write in sector order
+  const float* dens;       // NULL: no density weighting
+  float scale;             // 1 (kernel only) or 1/sqrt(Ng)
+  int n_coils;
+  bool fold_shift;         // read grid at FORWARD-fftshifted positions
+  bool image_grid;         // grid has image dims (DENSITY_ESTIMATION forward)
+};
+hipError_t launch_interp(const Geom& g, const InterpArgs& a, hipStream_t s);
+
+// prepost.hip
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_adjoint_post(const Geom& g, const float2* grid, int n_coils,
+                               const float* deapo, const float2* sens, float2* out,
+                               bool accumulate, int stop, hipStream_t s);
+hipError_t launch_forward_pre(const Geom& g, const float2* img, bool img_per_coil,
+                              const float2* sens, const float* deapo, float2* grid, int n_coils,
+                              hipStream_t s);
+hipError_t launch_image_prep(const Geom& g, const float2* img, bool img_per_coil,
+                             const float2* sens, const float* deapo, float2* out, int n_coils,
+                             hipStream_t s);
+
+}  // namespace gnufft

@@ -1,0 +1,173 @@
+/*
+ * gpunufft_hip.h — C ABI of the MI355X sector-NUFFT library (libgpunufft_hip.so).
+ *
+ * Plain pointers and sizes only; every entry point returns a status code
+ * (GNUFFT_OK on success) and never exits the process. gnufft_last_error()
+ * returns the calling thread's last error message.
+ *
+ * Data conventions (identical to the reference's):
+ *   trajectory  float SoA [x0..x(M-1), y..., (z...)], coordinates in [-0.5, 0.5]
+ *   k-space     complex64 (float re, im) [coil][sample], caller's sample order
+ *   image/sens  complex64 [coil][z][y][x], x fastest (z absent in 2-D)
+ *   grid        complex64 [coil][Gz][Gy][Gx]
+ * "stream" arguments are hipStream_t values passed as void* (NULL = default
+ * stream). Device pointers must be device memory of the current device.
+ *
+ * Each entry point below names the reference interface it replaces
+ * (paths relative to the reference tree).
+ */
+#ifndef GPUNUFFT_HIP_H
+#define GPUNUFFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GNUFFT_ABI_VERSION 1
+
+/* status codes */
+enum {
+  GNUFFT_OK = 0,
+  GNUFFT_ERR_INVALID_ARGUMENT = 1, /* -> std::invalid_argument in the C++ layer */
+  GNUFFT_ERR_OUT_OF_MEMORY = 2,    /* -> std::runtime_error (memory check) */
+  GNUFFT_ERR_DEVICE = 3,           /* HIP runtime error (reference: HANDLE_ERROR exit) */
+  GNUFFT_ERR_FFT = 4,              /* rocFFT error */
+  GNUFFT_ERR_UNSUPPORTED = 5,
+  GNUFFT_ERR_STATE = 6             /* call order violated (e.g. no trajectory yet) */
+};
+
+/* kernel lookup modes: CUDA/inc/gpuNUFFT_types.hpp:167-177 */
+enum {
+  GNUFFT_LOOKUP_NEAREST = 0, /* CONST_LOOKUP: nearest entry of the 1-D r^2 table */
+  GNUFFT_LOOKUP_LINEAR = 1   /* TEXTURE2D_LOOKUP: emulated bilinear texture fetch */
+};
+
+/* stop stages: GpuNUFFTOutput, CUDA/inc/gpuNUFFT_types.hpp:184-193 */
+enum {
+  GNUFFT_STOP_CONVOLUTION = 0,
+  GNUFFT_STOP_FFT = 1,
+  GNUFFT_STOP_DEAPODIZATION = 2,
+  GNUFFT_STOP_DENSITY_ESTIMATION = 3
+};
+
+/* memory location of caller buffers */
+enum { GNUFFT_HOST = 0, GNUFFT_DEVICE = 1 };
+
+typedef struct gnufft_plan_s* gnufft_plan;
+
+/* Derived geometry of a plan: GpuNUFFTInfo, CUDA/inc/gpuNUFFT_types.hpp:230-311 */
+typedef struct {
+  int32_t dim;              /* 2 or 3 */
+  uint32_t img[3];          /* image dims (z = 0 in 2-D) */
+  uint32_t grid[3];         /* oversampled grid dims (z = 0 in 2-D) */
+  uint32_t sectors[3];      /* sectors per dim (z = 1 in 2-D) */
+  uint32_t sector_count;
+  uint32_t sector_width;
+  uint32_t kernel_width;
+  uint32_t kernel_count;    /* entries of the lookup table the kernels index */
+  uint32_t sector_pad_width;
+  uint32_t sector_offset;
+  uint32_t n_samples;       /* M, 0 before a trajectory is set */
+  uint32_t n_sens_coils;    /* coils of the stored sensitivities, 0 = none */
+  uint32_t n_chunks;        /* gridding work items (sector chunks) */
+  int32_t lookup;
+  int32_t has_dens;
+  float osf;
+  float radius_sqr;
+  float dist_multiplier;
+  uint64_t grid_count;      /* Gx*Gy*max(Gz,1) */
+  uint64_t img_count;
+} gnufft_plan_info;
+
+/* ---- errors / version ---------------------------------------------------- */
+const char* gnufft_last_error(void);
+int gnufft_abi_version(void);
+
+/* ---- device memory helpers for callers built without HIP headers ---------
+ * replace the allocate/copy wrappers of CUDA/inc/cuda_utils.hpp:32-170 */
+int gnufft_malloc(void** ptr, size_t bytes);
+int gnufft_free(void* ptr);
+int gnufft_memcpy(void* dst, const void* src, size_t bytes, int dst_loc, int src_loc,
+                  void* stream);
+int gnufft_zero(void* ptr, size_t bytes, void* stream); /* fill kernel, graph-capture safe */
+int gnufft_stream_synchronize(void* stream);
+int gnufft_mem_info(size_t* free_bytes, size_t* total_bytes); /* checkMemoryConsumption,
+                                     CUDA/src/gpuNUFFT_operator_factory.cpp:628-663 */
+
+/* ---- plan (one operator) ---------------------------------------------------
+ * Replaces GpuNUFFTOperator construction + initKernel + initGpuNUFFTInfo
+ * (CUDA/src/gpuNUFFT_operator.cpp:43-146) and the deapodization
+ * precompute (CUDA/src/gpuNUFFT_operator_factory.cpp:359-448).
+ * img_dims[2] == 0 selects 2-D. */
+int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t kernel_width,
+                       uint32_t sector_width, float osf, int lookup, void* stream);
+int gnufft_plan_destroy(gnufft_plan plan);
+int gnufft_plan_get_info(gnufft_plan plan, gnufft_plan_info* info);
+
+/* Sector precompute on the device: assignSectors + sort + sectorDataCount +
+ * sorted trajectory/density (CUDA/src/gpuNUFFT_operator_factory.cpp:128-205,
+ * :450-553; CUDA/src/gpu/precomp_kernels.cu:8-177). traj/dens live at
+ * `location`; dens may be NULL. Samples are ordered by (sector, index):
+ * the stable order of the reference's std::sort. */
+int gnufft_plan_set_trajectory(gnufft_plan plan, const float* traj, uint32_t n_samples,
+                               const float* dens, int location, void* stream);
+
+/* Load precomputed tables (loadPrecomputedGpuNUFFTOperator,
+ * CUDA/src/gpuNUFFT_operator_factory.cpp:576-626). traj_sorted is SoA in
+ * sector order; dens_sorted and deapo may be NULL (deapo NULL: computed). */
+int gnufft_plan_load_precomputed(gnufft_plan plan, const float* traj_sorted,
+                                 const uint32_t* data_indices, const uint32_t* sector_data_count,
+                                 const uint32_t* sector_centers, const float* dens_sorted,
+                                 const float* deapo, uint32_t n_samples, int location,
+                                 void* stream);
+
+/* Coil sensitivities [coil][img], kept on the device (setSens,
+ * CUDA/inc/gpuNUFFT_operator.hpp:113-116). NULL / 0 coils clears them. */
+int gnufft_plan_set_sens(gnufft_plan plan, const void* sens, uint32_t n_coils, int location,
+                         void* stream);
+
+/* Copy precompute tables to host buffers (any pointer may be NULL):
+ * getDataIndices, getSectorDataCount, getSectorCenters, getKSpaceTraj,
+ * getDens, deapodization function, getKernel
+ * (CUDA/inc/gpuNUFFT_operator.hpp:135-201). */
+int gnufft_plan_get_tables(gnufft_plan plan, uint32_t* data_indices,
+                           uint32_t* sector_data_count, uint32_t* sector_centers,
+                           float* traj_sorted, float* dens_sorted, float* deapo, float* lut);
+
+/* Sector id of every sample in the caller's order (assignSectors output). */
+int gnufft_plan_get_assigned_sectors(gnufft_plan plan, uint32_t* sectors_host);
+
+/* ---- operators ------------------------------------------------------------
+ * Adjoint: GpuNUFFTOperator::performGpuNUFFTAdj(GpuArray, GpuArray&, out),
+ * CUDA/src/gpuNUFFT_operator.cpp:370-580 (stage semantics of :615-861).
+ * kdata [n_coils][M] (caller order, device); out device:
+ *   CONVOLUTION / DENSITY_ESTIMATION: [n_coils][grid]
+ *   FFT: [n_coils][img] (cropped, scaled, before deapodization)
+ *   DEAPODIZATION: [1][img] with sensitivities (coil-combined), else [n_coils][img] */
+int gnufft_adjoint(gnufft_plan plan, const void* kdata, void* out, uint32_t n_coils, int stop,
+                   void* stream);
+
+/* Forward: performForwardGpuNUFFT(GpuArray, GpuArray&, out),
+ * CUDA/src/gpuNUFFT_operator.cpp:895-1059 (stage semantics of :1092-1268).
+ * img device [1][img] with sensitivities (expanded to n_coils = sens coils)
+ * or [n_coils][img]; out device [n_coils][M] in the caller's order. */
+int gnufft_forward(gnufft_plan plan, const void* img, void* out, uint32_t n_coils, int stop,
+                   void* stream);
+
+/* ---- gridding kernels alone (sorted sample order, no shift) ----------------
+ * adjConvolution / forwardConvolution, CUDA/src/gpuNUFFT_operator.cpp:166-184:
+ * spread: data_sorted [n_coils][M] -> grid [n_coils][grid] (grid is zeroed first)
+ * interp: grid [n_coils][grid] -> data_sorted [n_coils][M] */
+int gnufft_spread_sorted(gnufft_plan plan, const void* data_sorted, void* grid,
+                         uint32_t n_coils, void* stream);
+int gnufft_interp_sorted(gnufft_plan plan, const void* grid, void* data_sorted,
+                         uint32_t n_coils, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUNUFFT_HIP_H */

@@ -1,0 +1,258 @@
+"""HIP path (through the C ABI) against the oracle on identical seeded inputs.
+
+Tolerances: integer tables bit-exact; lookup table bit-exact; floating-point
+outputs rel-L2 <= 1e-5 (atomics reorder float sums)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from gpunufft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a ROCm device"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from gpunufft_amd import capi as C
+    C.load()
+    return C
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+CASES = [
+    # name, traj, img_dims, osf, kw, sw
+    ("radial2d_64", lambda: W.radial_2d(64, 128), (64, 64), 2.0, 3, 8),
+    ("golden2d_aniso", lambda: W.radial_2d(40, 96, golden=True), (48, 32), 2.0, 3, 8),
+    ("rand2d_odd_grid", lambda: np.random.default_rng(5).uniform(-0.5, 0.5, (2, 3000)).astype(np.float32),
+     (37, 29), 1.5, 3, 8),
+    ("rand2d_kw5_sw5", lambda: np.random.default_rng(6).uniform(-0.5, 0.5, (2, 2000)).astype(np.float32),
+     (40, 40), 1.25, 5, 5),
+    ("koosh3d_small", lambda: W.koosh_3d(256, 40), (24, 24, 24), 1.5, 3, 8),
+    ("rand3d_aniso_odd", lambda: np.random.default_rng(7).uniform(-0.5, 0.5, (3, 4000)).astype(np.float32),
+     (20, 18, 11), 1.5, 3, 8),
+    ("rand3d_kw4_sw6", lambda: np.random.default_rng(8).uniform(-0.5, 0.5, (3, 1500)).astype(np.float32),
+     (16, 16, 16), 2.0, 4, 6),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_precompute_tables_bit_exact(oracle, torch_dev, capi, case):
+    name, tf, img, osf, kw, sw = case
+    traj = tf()
+    dens = W.ramp_density(traj)
+    print("case", name, flush=True)
+    pl = capi.Plan(img, kw, sw, osf)
+    pl.set_trajectory(dev(torch_dev, traj), dev(torch_dev, dens))
+    t = pl.tables()
+    ref = oracle.precompute(traj, img, osf, kw, sw, dens)
+    stable = oracle.precompute(traj, img, osf, kw, sw, dens, order="stable")
+    assert np.array_equal(pl.assigned_sectors(), ref["sectors"])
+    assert np.array_equal(t["sector_data_count"], ref["sector_data_count"])
+    assert np.array_equal(t["sector_centers"], ref["sector_centers"])
+    # canonical (stable) order; same per-sector index sets as the reference std::sort
+    assert np.array_equal(t["data_indices"], stable["data_indices"])
+    sdc = ref["sector_data_count"]
+    for s in np.nonzero(np.diff(sdc))[0][:200]:
+        a, b = sdc[s], sdc[s + 1]
+        assert set(t["data_indices"][a:b]) == set(ref["data_indices"][a:b])
+    assert np.array_equal(t["traj_sorted"], stable["traj_sorted"])
+    assert np.array_equal(t["dens_sorted"], stable["dens_sorted"])
+
+
+@pytest.mark.parametrize("lookup", [0, 1])
+@pytest.mark.parametrize("img,osf,kw", [((64, 64), 2.0, 3), ((32, 32, 32), 1.5, 3),
+                                        ((30, 20), 1.25, 5), ((16, 16, 16), 1.0, 4)])
+def test_lut_and_deapo(oracle, torch_dev, capi, img, osf, kw, lookup):
+    pl = capi.Plan(img, kw, 8, osf, lookup)
+    pl.set_trajectory(dev(torch_dev, np.zeros((len(img), 1), np.float32)))
+    t = pl.tables()
+    assert np.array_equal(t["lut"], oracle.load_lut(osf, kw, lookup))
+    ref = oracle.compute_deapo(img, kw, osf, lookup)
+    got = t["deapo"].reshape(ref.shape)
+    assert np.max(np.abs(got - ref) / np.abs(ref)) < 2e-5
+
+
+def _gridding_case(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, seed):
+    rng = np.random.default_rng(seed)
+    pl = capi.Plan(img, kw, sw, osf, lookup)
+    pl.set_trajectory(dev(torch_dev, traj))
+    pre = oracle.precompute(traj, img, osf, kw, sw, lookup=lookup, order="stable")
+    gi = pre["info"]
+    lut = oracle.load_lut(osf, kw, lookup)
+    M = traj.shape[1]
+    data = W.complex_normal(rng, (C, M))
+    g_ref = oracle.spread(gi, lookup, lut, pre["traj_sorted"], data, pre["sector_data_count"],
+                          pre["sector_centers"])
+    grid = torch_dev.empty((C,) + g_ref.shape[1:], dtype=torch_dev.complex64, device="cuda")
+    pl.spread_sorted(dev(torch_dev, data), grid, C)
+    assert oracle.rel_l2(host(grid), g_ref) < TOL
+    gin = W.complex_normal(rng, g_ref.shape)
+    s_ref = oracle.interp(gi, lookup, lut, pre["traj_sorted"], gin, pre["sector_data_count"],
+                          pre["sector_centers"])
+    out = torch_dev.empty((C, M), dtype=torch_dev.complex64, device="cuda")
+    pl.interp_sorted(dev(torch_dev, gin), out, C)
+    assert oracle.rel_l2(host(out), s_ref) < TOL
+
+
+@pytest.mark.parametrize("lookup", [0, 1])
+@pytest.mark.parametrize("C", [1, 3, 8, 11])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3], CASES[4], CASES[5], CASES[6]],
+                         ids=lambda c: c[0])
+def test_spread_interp_kernels(oracle, torch_dev, capi, case, C, lookup):
+    name, tf, img, osf, kw, sw = case
+    print("case", name, C, lookup, flush=True)
+    _gridding_case(oracle, torch_dev, capi, tf(), img, osf, kw, sw, C, lookup, seed=C)
+
+
+def test_heavy_sector_chunks(oracle, torch_dev, capi):
+    """Thousands of samples in one sector (split over many workgroups)."""
+    rng = np.random.default_rng(1)
+    traj = np.concatenate([rng.uniform(-0.01, 0.01, (3, 6000)),
+                           rng.uniform(-0.5, 0.5, (3, 500))], axis=1).astype(np.float32)
+    _gridding_case(oracle, torch_dev, capi, traj, (16, 16, 16), 2.0, 3, 8, 8, 0, seed=2)
+
+
+def _op_pair(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, dens, sens):
+    pl = capi.Plan(img, kw, sw, osf, lookup)
+    pl.set_trajectory(dev(torch_dev, traj), None if dens is None else dev(torch_dev, dens))
+    if sens is not None:
+        pl.set_sens(dev(torch_dev, sens), C)
+    op = oracle.Operator(traj, img, kw, sw, osf, dens=dens, sens=sens, lookup=lookup,
+                         order="stable")
+    return pl, op
+
+
+@pytest.mark.parametrize("lookup", [0, 1])
+@pytest.mark.parametrize("use_dens,use_sens", [(False, False), (True, False), (True, True)])
+@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2], CASES[4], CASES[5]],
+                         ids=lambda c: c[0])
+def test_operator_all_stages(oracle, torch_dev, capi, case, use_dens, use_sens, lookup):
+    name, tf, img, osf, kw, sw = case
+    print("case", name, use_dens, use_sens, lookup, flush=True)
+    traj = tf()
+    M = traj.shape[1]
+    C = 4
+    rng = np.random.default_rng(42)
+    dens = W.ramp_density(traj) if use_dens else None
+    sens = W.gaussian_sens(img, C, seed=3) if use_sens else None
+    pl, op = _op_pair(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, dens, sens)
+    gi = op.gi
+    kd = W.complex_normal(rng, (C, M))
+    ishape = oracle.img_shape(gi)
+    gshape = oracle.grid_shape(gi)
+    k_dev = dev(torch_dev, kd)
+    for stop in (oracle.CONVOLUTION, oracle.FFT, oracle.DEAPODIZATION):
+        ref = op.adjoint(kd, stop=stop)
+        if stop == oracle.CONVOLUTION:
+            shape = (C,) + gshape
+        elif stop == oracle.DEAPODIZATION and use_sens:
+            shape = (1,) + ishape
+        else:
+            shape = (C,) + ishape
+        if stop == oracle.FFT and use_sens:
+            ref = op.adjoint(kd, stop=stop)  # per coil
+        out = torch_dev.empty(shape, dtype=torch_dev.complex64, device="cuda")
+        pl.adjoint(k_dev, out, C, stop)
+        assert oracle.rel_l2(host(out), ref) < TOL, ("adjoint", stop)
+    x = W.complex_normal(rng, ((1,) if use_sens else (C,)) + ishape)
+    ref = op.forward(x)
+    out = torch_dev.empty((C, M), dtype=torch_dev.complex64, device="cuda")
+    pl.forward(dev(torch_dev, x), out, C)
+    assert oracle.rel_l2(host(out), ref) < TOL, "forward"
+
+
+def test_forward_density_estimation_osf1(oracle, torch_dev, capi):
+    traj = W.radial_2d(16, 32)
+    pl, op = _op_pair(oracle, torch_dev, capi, traj, (32, 32), 1.0, 3, 8, 2, 0, None, None)
+    x = W.complex_normal(np.random.default_rng(0), (2, 32, 32))
+    ref = op.forward(x, stop=oracle.DENSITY_ESTIMATION)
+    out = torch_dev.empty(ref.shape, dtype=torch_dev.complex64, device="cuda")
+    pl.forward(dev(torch_dev, x), out, 2, oracle.DENSITY_ESTIMATION)
+    assert oracle.rel_l2(host(out), ref) < TOL
+
+
+def test_reference_fixtures_on_gpu(oracle, fixtures, torch_dev, capi):
+    f = fixtures["forward_2d_32_4"]
+    c = np.array(f["coords_flat"], np.float32).reshape(2, 3)
+    img = np.full((f["coils"], 32, 32), 1 + 1j, np.complex64)
+    for lk, key in [(0, "expect_nearest"), (1, "expect_texture")]:
+        pl = capi.Plan(f["img"], f["kw"], f["sw"], f["osf"], lk)
+        pl.set_trajectory(dev(torch_dev, c))
+        out = torch_dev.empty((f["coils"], 3), dtype=torch_dev.complex64, device="cuda")
+        pl.forward(dev(torch_dev, img), out, f["coils"])
+        y = host(out)
+        assert np.all(np.abs(y[:, 0].real - f[key]) < f["eps"])
+        assert np.all(np.abs(y[:, 0] - y[0, 0]) < 1e-5)
+    f = fixtures["deapo_2d_30_b1"]
+    C = f["coils"]
+    pl = capi.Plan(f["img"], f["kw"], f["sw"], f["osf"], 0)
+    pl.set_trajectory(dev(torch_dev, np.zeros((2, 1), np.float32)),
+                      dev(torch_dev, np.ones(1, np.float32)))
+    pl.set_sens(dev(torch_dev, np.full((C, 32, 32), complex(*f["sens"]), np.complex64)), C)
+    out = torch_dev.empty((1, 32, 32), dtype=torch_dev.complex64, device="cuda")
+    pl.adjoint(dev(torch_dev, np.full((C, 1), complex(*f["data"]), np.complex64)), out, C)
+    assert abs(host(out)[0, 0, 0].real - f["expect"]) < f["eps"]
+
+
+def _adjointness(torch_dev, pl, img_shape, C, M, use_sens, seed):
+    torch = torch_dev
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    xs = (1,) if use_sens else (C,)
+    x = torch.randn(xs + img_shape, dtype=torch.complex64, device="cuda", generator=g)
+    y = torch.randn((C, M), dtype=torch.complex64, device="cuda", generator=g)
+    Ax = torch.empty((C, M), dtype=torch.complex64, device="cuda")
+    AHy = torch.empty(xs + img_shape, dtype=torch.complex64, device="cuda")
+    pl.forward(x, Ax, C)
+    pl.adjoint(y, AHy, C)
+    lhs = torch.vdot(Ax.flatten().to(torch.complex128), y.flatten().to(torch.complex128))
+    rhs = torch.vdot(x.flatten().to(torch.complex128), AHy.flatten().to(torch.complex128))
+    return abs(complex(lhs - rhs)) / abs(complex(lhs))
+
+
+@pytest.mark.parametrize("cfg", ["radial2d_256_mc16", "koosh3d_128_mc8"])
+def test_full_size_properties(torch_dev, capi, cfg):
+    """Full BASELINE sizes: adjointness and linearity (size-independent checks)."""
+    w = W.config(cfg)
+    print("config", cfg, flush=True)
+    torch = torch_dev
+    traj, img, C = w["traj"], w["img_dims"], w["coils"]
+    M = traj.shape[1]
+    pl = capi.Plan(img, w["kw"], w["sw"], w["osf"], 0)
+    pl.set_trajectory(dev(torch, traj), dev(torch, w["dens"]))
+    ishape = tuple(img[::-1])
+    err = _adjointness(torch, pl, ishape, C, M, False, 1)
+    assert err < 1e-5, err
+    sens = W.gaussian_sens(img, C, seed=w["seed"])
+    pl.set_sens(dev(torch, sens), C)
+    err = _adjointness(torch, pl, ishape, C, M, True, 2)
+    assert err < 1e-5, err
+    # linearity of the adjoint
+    g = torch.Generator(device="cuda").manual_seed(3)
+    y1 = torch.randn((C, M), dtype=torch.complex64, device="cuda", generator=g)
+    y2 = torch.randn((C, M), dtype=torch.complex64, device="cuda", generator=g)
+    a, b = 0.7 - 0.2j, -1.3 + 0.5j
+    o1 = torch.empty((1,) + ishape, dtype=torch.complex64, device="cuda")
+    o2, o3 = torch.empty_like(o1), torch.empty_like(o1)
+    pl.adjoint(y1, o1, C)
+    pl.adjoint(y2, o2, C)
+    pl.adjoint((a * y1 + b * y2).contiguous(), o3, C)
+    lin = torch.linalg.vector_norm(o3 - (a * o1 + b * o2)) / torch.linalg.vector_norm(o3)
+    assert float(lin) < 1e-5
+    torch.cuda.synchronize()
