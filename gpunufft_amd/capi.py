@@ -26,8 +26,12 @@ EXPORTS = [
     "gnufft_plan_destroy", "gnufft_plan_get_info", "gnufft_plan_set_trajectory",
     "gnufft_plan_load_precomputed", "gnufft_plan_set_sens", "gnufft_plan_get_tables",
     "gnufft_plan_get_assigned_sectors", "gnufft_adjoint", "gnufft_forward",
-    "gnufft_spread_sorted", "gnufft_interp_sorted",
+    "gnufft_spread_sorted", "gnufft_interp_sorted", "gnufft_plan_set_timing",
+    "gnufft_plan_get_timing",
 ]
+
+T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
+T_NAMES = ["spread", "interp", "fft", "zero", "prepost"]
 
 
 class GnufftError(RuntimeError):
@@ -59,6 +63,10 @@ class PlanInfo(ctypes.Structure):
         ("grid_count", ctypes.c_uint64),
         ("img_count", ctypes.c_uint64),
     ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 5), ("calls", ctypes.c_uint32 * 5)]
 
 
 _lib = None
@@ -95,6 +103,8 @@ def load(path=LIB_PATH):
     L.gnufft_forward.argtypes = [vp, vp, vp, u32, i32, vp]
     L.gnufft_spread_sorted.argtypes = [vp, vp, vp, u32, vp]
     L.gnufft_interp_sorted.argtypes = [vp, vp, vp, u32, vp]
+    L.gnufft_plan_set_timing.argtypes = [vp, i32]
+    L.gnufft_plan_get_timing.argtypes = [vp, ctypes.POINTER(Timing), i32]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or i32
     L.gnufft_last_error.restype = ctypes.c_char_p
@@ -225,6 +235,15 @@ class Plan:
         check(load().gnufft_spread_sorted(self._h, _ptr_loc(data_sorted)[0], _ptr_loc(grid)[0],
                                           n_coils, _stream_ptr(stream)))
         return grid
+
+    def set_timing(self, enable=True):
+        check(load().gnufft_plan_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self, reset=True):
+        """{stage: (total_ms, calls)} since the last reset (synchronises)."""
+        t = Timing()
+        check(load().gnufft_plan_get_timing(self._h, ctypes.byref(t), 1 if reset else 0))
+        return {T_NAMES[i]: (t.ms[i], t.calls[i]) for i in range(5)}
 
     def interp_sorted(self, grid, data_sorted, n_coils, stream=None):
         check(load().gnufft_interp_sorted(self._h, _ptr_loc(grid)[0], _ptr_loc(data_sorted)[0],

@@ -123,6 +123,45 @@ struct FftPlans {
   size_t work_bytes = 0;
 };
 
+// HIP-event stage timer of one plan.
+struct StageTimer {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[GNUFFT_T_COUNT];
+  double ms[GNUFFT_T_COUNT] = {0, 0, 0, 0, 0};
+  uint32_t calls[GNUFFT_T_COUNT] = {0, 0, 0, 0, 0};
+
+  hipEvent_t begin(int cat, hipStream_t s) {
+    if (!on) return nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return nullptr;
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return nullptr;
+    }
+    (void)hipEventRecord(a, s);
+    ev[cat].push_back({a, b});
+    return b;
+  }
+  void end(hipEvent_t b, hipStream_t s) {
+    if (b) (void)hipEventRecord(b, s);
+  }
+  void collect() {
+    for (int c = 0; c < GNUFFT_T_COUNT; c++) {
+      for (auto& e : ev[c]) {
+        float t = 0.0f;
+        if (hipEventSynchronize(e.second) == hipSuccess &&
+            hipEventElapsedTime(&t, e.first, e.second) == hipSuccess) {
+          ms[c] += t;
+          calls[c]++;
+        }
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
+      ev[c].clear();
+    }
+  }
+};
+
 }  // namespace
 
 struct gnufft_plan_s {
@@ -157,7 +196,17 @@ struct gnufft_plan_s {
   float2* img_tmp_d = nullptr;
   uint32_t img_tmp_coils = 0;
   std::map<uint32_t, FftPlans> ffts;
+  StageTimer timer;
 };
+
+// Runs `expr` (a hipError_t launch) bracketed by the plan's stage timer.
+#define TIMED(plan, cat, stream, expr)                         \
+  do {                                                         \
+    hipEvent_t t_end_ = (plan)->timer.begin((cat), (stream));  \
+    hipError_t t_e_ = (expr);                                  \
+    (plan)->timer.end(t_end_, (stream));                       \
+    HIP_TRY(t_e_);                                             \
+  } while (0)
 
 namespace {
 
@@ -336,7 +385,10 @@ int run_fft(gnufft_plan p, uint32_t batch, bool inverse, void* buf, hipStream_t 
   if (rc) return rc;
   ROCFFT_TRY(rocfft_execution_info_set_stream(f->info, s));
   void* bufs[1] = {buf};
-  ROCFFT_TRY(rocfft_execute(inverse ? f->inv : f->fwd, bufs, nullptr, f->info));
+  hipEvent_t t_end = p->timer.begin(GNUFFT_T_FFT, s);
+  rocfft_status st = rocfft_execute(inverse ? f->inv : f->fwd, bufs, nullptr, f->info);
+  p->timer.end(t_end, s);
+  ROCFFT_TRY(st);
   return GNUFFT_OK;
 }
 
@@ -516,9 +568,30 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   return GNUFFT_OK;
 }
 
+int gnufft_plan_set_timing(gnufft_plan p, int enable) {
+  if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_timing: null plan");
+  p->timer.on = enable != 0;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_get_timing(gnufft_plan p, gnufft_timing* out, int reset) {
+  if (!p || !out) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_timing: null argument");
+  p->timer.collect();
+  for (int c = 0; c < GNUFFT_T_COUNT; c++) {
+    out->ms[c] = p->timer.ms[c];
+    out->calls[c] = p->timer.calls[c];
+    if (reset) {
+      p->timer.ms[c] = 0.0;
+      p->timer.calls[c] = 0;
+    }
+  }
+  return GNUFFT_OK;
+}
+
 int gnufft_plan_destroy(gnufft_plan p) {
   if (!p) return GNUFFT_OK;
   (void)hipDeviceSynchronize();
+  p->timer.collect();
   free_traj(p);
   free_ffts(p);
   dfree(p->lut_d);
@@ -696,7 +769,7 @@ int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uin
   if (!p || !grid || (!data_sorted && p->M)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "spread_sorted: null");
   if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "spread_sorted: no trajectory set");
   hipStream_t s = (hipStream_t)stream;
-  HIP_TRY(launch_zero(grid, (size_t)n_coils * p->g.Ng * sizeof(float2), s));
+  TIMED(p, GNUFFT_T_ZERO, s, launch_zero(grid, (size_t)n_coils * p->g.Ng * sizeof(float2), s));
   SpreadArgs a{};
   a.data = (const float2*)data_sorted;
   a.coil_stride = p->M;
@@ -708,7 +781,7 @@ int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uin
   a.grid = (float2*)grid;
   a.n_coils = (int)n_coils;
   a.fold_shift = false;
-  HIP_TRY(launch_spread(p->g, a, s));
+  TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, s));
   return GNUFFT_OK;
 }
 
@@ -727,7 +800,7 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   a.coil_stride = p->M;
   a.scale = 1.0f;
   a.n_coils = (int)n_coils;
-  HIP_TRY(launch_interp(p->g, a, (hipStream_t)stream));
+  TIMED(p, GNUFFT_T_INTERP, (hipStream_t)stream, launch_interp(p->g, a, (hipStream_t)stream));
   return GNUFFT_OK;
 }
 
@@ -751,7 +824,7 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
   for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
     const uint32_t nc = std::min(cc, n_coils - c0);
     float2* grid = grid_out ? (float2*)out + (size_t)c0 * g.Ng : p->grid_d;
-    HIP_TRY(launch_zero(grid, (size_t)nc * g.Ng * sizeof(float2), s));
+    TIMED(p, GNUFFT_T_ZERO, s, launch_zero(grid, (size_t)nc * g.Ng * sizeof(float2), s));
     SpreadArgs a{};
     a.data = (const float2*)kdata + (size_t)c0 * p->M;
     a.coil_stride = p->M;
@@ -765,15 +838,15 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
     a.grid = grid;
     a.n_coils = (int)nc;
     a.fold_shift = !grid_out;
-    HIP_TRY(launch_spread(g, a, s));
+    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, s));
     if (grid_out) continue;
     int rc = run_fft(p, nc, true, grid, s);
     if (rc) return rc;
     if (use_sens) {
-      HIP_TRY(launch_adjoint_post(g, grid, (int)nc, p->deapo_d, p->sens_d + (size_t)c0 * g.Ni,
+      TIMED(p, GNUFFT_T_PREPOST, s, launch_adjoint_post(g, grid, (int)nc, p->deapo_d, p->sens_d + (size_t)c0 * g.Ni,
                                   (float2*)out, c0 > 0, stop, s));
     } else {
-      HIP_TRY(launch_adjoint_post(g, grid, (int)nc, p->deapo_d, nullptr,
+      TIMED(p, GNUFFT_T_PREPOST, s, launch_adjoint_post(g, grid, (int)nc, p->deapo_d, nullptr,
                                   (float2*)out + (size_t)c0 * g.Ni, false, stop, s));
     }
   }
@@ -803,7 +876,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
       HIP_TRY(hipMalloc(&p->img_tmp_d, (size_t)n_coils * g.Ni * sizeof(float2)));
       p->img_tmp_coils = n_coils;
     }
-    HIP_TRY(launch_image_prep(g, (const float2*)img, !use_sens, p->sens_d, p->deapo_d,
+    TIMED(p, GNUFFT_T_PREPOST, s, launch_image_prep(g, (const float2*)img, !use_sens, p->sens_d, p->deapo_d,
                               p->img_tmp_d, (int)n_coils, s));
     InterpArgs a{};
     a.grid = p->img_tmp_d;
@@ -817,7 +890,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.scatter = p->idx_d;
     a.scale = 1.0f;
     a.n_coils = (int)n_coils;
-    HIP_TRY(launch_interp(g, a, s));
+    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, s));
     return GNUFFT_OK;
   }
   const uint32_t cc = coil_batch(p, n_coils);
@@ -826,7 +899,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
   for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
     const uint32_t nc = std::min(cc, n_coils - c0);
     const float2* src = use_sens ? (const float2*)img : (const float2*)img + (size_t)c0 * g.Ni;
-    HIP_TRY(launch_forward_pre(g, src, !use_sens, use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr,
+    TIMED(p, GNUFFT_T_PREPOST, s, launch_forward_pre(g, src, !use_sens, use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr,
                                p->deapo_d, p->grid_d, (int)nc, s));
     rc = run_fft(p, nc, false, p->grid_d, s);
     if (rc) return rc;
@@ -844,7 +917,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.scale = g.fft_scale;
     a.n_coils = (int)nc;
     a.fold_shift = true;
-    HIP_TRY(launch_interp(g, a, s));
+    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, s));
   }
   return GNUFFT_OK;
 }

@@ -157,6 +157,26 @@ int gnufft_adjoint(gnufft_plan plan, const void* kdata, void* out, uint32_t n_co
 int gnufft_forward(gnufft_plan plan, const void* img, void* out, uint32_t n_coils, int stop,
                    void* stream);
 
+/* ---- per-stage timing --------------------------------------------------------
+ * Replaces the reference's DEBUG stage timing (startTiming/stopTiming,
+ * CUDA/src/gpuNUFFT_operator.cpp:1296-1311): when enabled, HIP events on the
+ * launch stream bracket every kernel stage of the plan's calls. Reading
+ * synchronises on the recorded events and returns totals since the last reset. */
+enum {
+  GNUFFT_T_SPREAD = 0,
+  GNUFFT_T_INTERP = 1,
+  GNUFFT_T_FFT = 2,
+  GNUFFT_T_ZERO = 3,
+  GNUFFT_T_PREPOST = 4,
+  GNUFFT_T_COUNT = 5
+};
+typedef struct {
+  double ms[GNUFFT_T_COUNT];
+  uint32_t calls[GNUFFT_T_COUNT];
+} gnufft_timing;
+int gnufft_plan_set_timing(gnufft_plan plan, int enable);
+int gnufft_plan_get_timing(gnufft_plan plan, gnufft_timing* out, int reset);
+
 /* ---- gridding kernels alone (sorted sample order, no shift) ----------------
  * adjConvolution / forwardConvolution, CUDA/src/gpuNUFFT_operator.cpp:166-184:
  * spread: data_sorted [n_coils][M] -> grid [n_coils][grid] (grid is zeroed first)
