@@ -80,6 +80,13 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError("libgpunufft_hip.so is not built (%s); run __graft_entry__.build()"
                            % path)
+    # torch ships its own HIP runtime; loading it first makes the library
+    # bind to that one, so the process holds a single runtime and device
+    # pointers from torch tensors are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     vp, u32, i32, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float
     u32p, f32p = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float)

@@ -531,6 +531,9 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   g.P = (int)sector_width + 2 * (int)std::floor((float)kernel_width / 2.0f);
   g.Pmax = g.P - 1;
   g.off = (int)std::floor((float)g.P / 2.0f);
+  g.mP = udiv_magic((uint32_t)g.P);
+  g.mP2 = udiv_magic((uint32_t)(g.P * g.P));
+  g.mPD = udiv_magic((uint32_t)(g.dim == 3 ? g.P * g.P * g.P : g.P * g.P));
   g.lookup = lookup;
   const long K = lut_size(osf, kernel_width, lookup);
   if (K < 2 || K > 100000) {
@@ -551,7 +554,7 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   g.Ni = (long long)g.N[0] * g.N[1] * g.N[2];
   g.fft_scale = 1.0f / (float)std::sqrt((float)g.Ng);
   g.M = 0;
-  p->chunk = g.dim == 3 ? 512 : 256;
+  p->chunk = 256;
 
   p->lut = make_lut(K, kernel_width, osf);
   int rc = copy_in((void**)&p->lut_d, p->lut.data(), p->lut.size() * sizeof(float), GNUFFT_HOST, s);
@@ -781,7 +784,7 @@ int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uin
   a.grid = (float2*)grid;
   a.n_coils = (int)n_coils;
   a.fold_shift = false;
-  TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, s));
+  TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, (int)p->chunk, s));
   return GNUFFT_OK;
 }
 
@@ -800,7 +803,7 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   a.coil_stride = p->M;
   a.scale = 1.0f;
   a.n_coils = (int)n_coils;
-  TIMED(p, GNUFFT_T_INTERP, (hipStream_t)stream, launch_interp(p->g, a, (hipStream_t)stream));
+  TIMED(p, GNUFFT_T_INTERP, (hipStream_t)stream, launch_interp(p->g, a, (int)p->chunk, (hipStream_t)stream));
   return GNUFFT_OK;
 }
 
@@ -838,7 +841,7 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
     a.grid = grid;
     a.n_coils = (int)nc;
     a.fold_shift = !grid_out;
-    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, s));
+    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, (int)p->chunk, s));
     if (grid_out) continue;
     int rc = run_fft(p, nc, true, grid, s);
     if (rc) return rc;
@@ -890,7 +893,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.scatter = p->idx_d;
     a.scale = 1.0f;
     a.n_coils = (int)n_coils;
-    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, s));
+    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->chunk, s));
     return GNUFFT_OK;
   }
   const uint32_t cc = coil_batch(p, n_coils);
@@ -917,7 +920,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.scale = g.fft_scale;
     a.n_coils = (int)nc;
     a.fold_shift = true;
-    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, s));
+    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->chunk, s));
   }
   return GNUFFT_OK;
 }

@@ -36,7 +36,16 @@ struct Geom {
   long long Ni;     // image points per coil
   int M;            // samples
   float fft_scale;  // 1/sqrt(Ng)
+  // multiply-high reciprocals: q = umulhi(n, mP) == n / P for n < 2^20
+  uint32_t mP, mP2, mPD;  // for P, P*P and the padded sector point count
 };
+
+// ceil(2^32 / d): exact quotient via __umulhi(n, m) for n*d < 2^32.
+__host__ inline uint32_t udiv_magic(uint32_t d) {
+  return (uint32_t)((0x100000000ull + d - 1) / d);
+}
+
+__device__ inline uint32_t udiv(uint32_t n, uint32_t m) { return __umulhi(n, m); }
 
 // One gridding work item: samples [begin, end) of a sector.
 struct Chunk {
@@ -106,11 +115,30 @@ __device__ __host__ inline bool axis_weight(const Geom& g, const float* lut, int
   return true;
 }
 
+// Same weight with the tap's k-space position precomputed
+// (kp = map_grid_to_kspace(i, G, center, off), identical float value).
+__device__ __host__ inline bool axis_weight_kp(const Geom& g, const float* lut, int a, float kp,
+                                               float pos, float* w) {
+  float d = (kp - pos) * g.aniso[a];
+  d *= d;
+  if (g.lookup == 0) {
+    if (!(d < g.rsq)) return false;
+    int li = (int)roundf(d * g.dist_mult);
+    *w = lut[li < g.K - 1 ? li : g.K - 1];
+    return true;
+  }
+  *w = lerp_fetch(lut, g.K, d * g.rsq_inv);
+  return true;
+}
+
 #pragma clang fp contract(on)
 
+// Periodic wrap into [0, G). Callers pass coordinates within a few grid
+// lengths of the grid, so a short loop of +-G replaces a division.
 __device__ __host__ inline int wrap(int p, int G) {
-  p %= G;
-  return p < 0 ? p + G : p;
+  while (p >= G) p -= G;
+  while (p < 0) p += G;
+  return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -150,7 +178,7 @@ struct SpreadArgs {
   int n_coils;
   bool fold_shift;         // store at INVERSE-fftshifted positions
 };
-hipError_t launch_spread(const Geom& g, const SpreadArgs& a, hipStream_t s);
+hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s);
 
 struct InterpArgs {
   const float2* grid;      // [coil][Ng]
@@ -168,7 +196,7 @@ struct InterpArgs {
   bool fold_shift;         // read grid at FORWARD-fftshifted positions
   bool image_grid;         // grid has image dims (DENSITY_ESTIMATION forward)
 };
-hipError_t launch_interp(const Geom& g, const InterpArgs& a, hipStream_t s);
+hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s);
 
 // prepost.hip
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
