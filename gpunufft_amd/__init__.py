@@ -7,6 +7,15 @@
 
 import os
 
+# torch ships its own copy of the HIP runtime under another file name. When
+# torch is present it must be loaded first: the native libraries then bind
+# to that copy (same soname) and the process holds one runtime, so device
+# pointers from torch tensors are valid in the library and vice versa.
+try:  # pragma: no cover - depends on the environment
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 

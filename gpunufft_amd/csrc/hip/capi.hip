@@ -186,7 +186,15 @@ struct gnufft_plan_s {
   uint32_t* centers_d = nullptr;
   Chunk* chunks_d = nullptr;
   uint32_t n_chunks = 0;
-  uint32_t chunk = 512;
+  uint32_t chunk = 1024;
+  WorkList work{};
+  uint32_t* sector_chunks_d = nullptr;  // [S+1] first chunk of each sector
+  // per-axis padded-sector cover lists of the grid (fold)
+  uint32_t* cover_d = nullptr;
+  Cover cover{};
+  // spread sub-grids, [chunk][coil][padded point], capacity in coils
+  float2* scratch_d = nullptr;
+  uint32_t scratch_coils = 0;
   // sensitivities
   float2* sens_d = nullptr;
   uint32_t n_sens = 0;
@@ -222,11 +230,17 @@ void free_traj(gnufft_plan p) {
   dfree(p->sdc_d);
   dfree(p->centers_d);
   dfree(p->chunks_d);
+  dfree(p->sector_chunks_d);
+  dfree(p->scratch_d);
   p->traj_sorted_d = p->dens_sorted_d = nullptr;
   p->idx_d = p->assigned_d = p->sdc_d = p->centers_d = nullptr;
   p->chunks_d = nullptr;
+  p->sector_chunks_d = nullptr;
+  p->scratch_d = nullptr;
+  p->scratch_coils = 0;
   p->M = 0;
   p->n_chunks = 0;
+  p->work = WorkList{};
   p->g.M = 0;
 }
 
@@ -342,11 +356,34 @@ int ensure_grid(gnufft_plan p, uint32_t coils) {
 // Coils processed together: all of them unless the grid would take more
 // than half of the free device memory (coil chunking,
 // CUDA/src/gpuNUFFT_operator.cpp:327-368).
+size_t padded_points(const Geom& g) {
+  return g.dim == 3 ? (size_t)g.P * g.P * g.P : (size_t)g.P * g.P;
+}
+
+size_t scratch_per_coil(gnufft_plan p) {
+  return (size_t)p->n_chunks * padded_points(p->g) * sizeof(float2);
+}
+
+int ensure_scratch(gnufft_plan p, uint32_t coils) {
+  if (p->scratch_coils >= coils) return GNUFFT_OK;
+  dfree(p->scratch_d);
+  p->scratch_d = nullptr;
+  p->scratch_coils = 0;
+  const size_t bytes = (size_t)coils * scratch_per_coil(p);
+  if (bytes == 0) return GNUFFT_OK;
+  if (hipMalloc(&p->scratch_d, bytes) != hipSuccess)
+    return fail(GNUFFT_ERR_OUT_OF_MEMORY, "spread: sub-grid scratch allocation");
+  p->scratch_coils = coils;
+  return GNUFFT_OK;
+}
+
 uint32_t coil_batch(gnufft_plan p, uint32_t n_coils) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 1;
-  size_t per = (size_t)p->g.Ng * sizeof(float2) * 2;  // grid + FFT work
-  size_t avail = fr / 2 + (size_t)p->grid_coils * p->g.Ng * sizeof(float2);
+  // grid + FFT work + spread sub-grids
+  size_t per = (size_t)p->g.Ng * sizeof(float2) * 2 + scratch_per_coil(p);
+  size_t avail = fr / 2 + (size_t)p->grid_coils * p->g.Ng * sizeof(float2) +
+                 (size_t)p->scratch_coils * scratch_per_coil(p);
   size_t cc = per ? avail / per : n_coils;
   if (cc < 1) cc = 1;
   return (uint32_t)std::min<size_t>(cc, n_coils);
@@ -406,10 +443,12 @@ int copy_in(void** dst, const void* src, size_t bytes, int location, hipStream_t
 int build_chunks(gnufft_plan p, hipStream_t s) {
   const uint32_t S = p->S;
   uint32_t* off_d = nullptr;
+  uint32_t* cnt_d = nullptr;
   void* temp = nullptr;
   size_t temp_bytes = 0;
-  HIP_TRY(hipMalloc(&off_d, 2 * (S + 1) * sizeof(uint32_t)));
-  uint32_t* cnt_d = off_d + (S + 1);
+  HIP_TRY(hipMalloc(&off_d, (S + 1) * sizeof(uint32_t)));
+  p->sector_chunks_d = off_d;  // kept: first chunk of each sector (fold)
+  HIP_TRY(hipMalloc(&cnt_d, (S + 1) * sizeof(uint32_t)));
   hipError_t e = count_chunks(p->sdc_d, S, p->chunk, cnt_d, off_d, nullptr, &temp_bytes, s);
   if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 4);
   if (e == hipSuccess) e = count_chunks(p->sdc_d, S, p->chunk, cnt_d, off_d, temp, &temp_bytes, s);
@@ -419,15 +458,40 @@ int build_chunks(gnufft_plan p, hipStream_t s) {
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e == hipSuccess) e = hipMalloc(&p->chunks_d, (size_t)(total ? total : 1) * sizeof(Chunk));
   if (e == hipSuccess) e = launch_fill_chunks(p->sdc_d, off_d, S, p->chunk, p->chunks_d, s);
+  std::vector<Chunk> h(total);
+  if (e == hipSuccess && total)
+    e = hipMemcpyAsync(h.data(), p->chunks_d, (size_t)total * sizeof(Chunk), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  dfree(off_d);
+  dfree(cnt_d);
   dfree(temp);
   if (e != hipSuccess) return fail(GNUFFT_ERR_DEVICE, std::string("chunk list: ") + hipGetErrorString(e));
   p->n_chunks = total;
+  std::vector<uint32_t> counts(total);
+  for (uint32_t i = 0; i < total; i++) counts[i] = h[i].end - h[i].begin;
+  p->work.chunks = p->chunks_d;
+  partition_work(counts.data(), total, &p->work);
   return GNUFFT_OK;
 }
 
 bool valid_stop(int stop) { return stop >= 0 && stop <= 3; }
+
+// Spread arguments shared by every entry point: samples in sector order
+// (no gather, no density) unless the caller sets them.
+SpreadArgs spread_args(gnufft_plan p, const float2* data, float2* grid, uint32_t n_coils) {
+  SpreadArgs a{};
+  a.data = data;
+  a.coil_stride = p->M;
+  a.crds = p->traj_sorted_d;
+  a.work = p->work;
+  a.centers = p->centers_d;
+  a.lut = p->lut_d;
+  a.scratch = p->scratch_d;
+  a.sector_chunks = p->sector_chunks_d;
+  a.cover = p->cover;
+  a.grid = grid;
+  a.n_coils = (int)n_coils;
+  return a;
+}
 
 }  // namespace
 
@@ -531,6 +595,10 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   g.P = (int)sector_width + 2 * (int)std::floor((float)kernel_width / 2.0f);
   g.Pmax = g.P - 1;
   g.off = (int)std::floor((float)g.P / 2.0f);
+  if (g.P > 255) {
+    delete p;
+    return fail(GNUFFT_ERR_UNSUPPORTED, "plan_create: padded sector width above 255");
+  }
   g.mP = udiv_magic((uint32_t)g.P);
   g.mP2 = udiv_magic((uint32_t)(g.P * g.P));
   g.mPD = udiv_magic((uint32_t)(g.dim == 3 ? g.P * g.P * g.P : g.P * g.P));
@@ -554,7 +622,7 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   g.Ni = (long long)g.N[0] * g.N[1] * g.N[2];
   g.fft_scale = 1.0f / (float)std::sqrt((float)g.Ng);
   g.M = 0;
-  p->chunk = 256;
+  p->chunk = 1024;
 
   p->lut = make_lut(K, kernel_width, osf);
   int rc = copy_in((void**)&p->lut_d, p->lut.data(), p->lut.size() * sizeof(float), GNUFFT_HOST, s);
@@ -563,6 +631,19 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
     if (e != hipSuccess) rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, "plan_create: deapo allocation");
   }
   if (rc == GNUFFT_OK) rc = compute_deapo(p, s);
+  if (rc == GNUFFT_OK) {
+    std::vector<uint32_t> words;
+    size_t base[6];
+    build_cover(g, &words, base);
+    rc = copy_in((void**)&p->cover_d, words.data(), words.size() * sizeof(uint32_t), GNUFFT_HOST, s);
+    if (rc == GNUFFT_OK) {
+      for (int a = 0; a < 3; a++) {
+        p->cover.off[a] = p->cover_d + base[a];
+        p->cover.ent[a] = p->cover_d + base[3 + a];
+      }
+      if (hipStreamSynchronize(s) != hipSuccess) rc = fail(GNUFFT_ERR_DEVICE, "plan_create: cover lists");
+    }
+  }
   if (rc != GNUFFT_OK) {
     gnufft_plan_destroy(p);
     return rc;
@@ -599,6 +680,7 @@ int gnufft_plan_destroy(gnufft_plan p) {
   free_ffts(p);
   dfree(p->lut_d);
   dfree(p->deapo_d);
+  dfree(p->cover_d);
   dfree(p->sens_d);
   dfree(p->grid_d);
   dfree(p->img_tmp_d);
@@ -716,6 +798,24 @@ int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
                            s));
   if (!rc) {
     HIP_TRY(hipStreamSynchronize(s));
+    // The gridding kernels derive padded-sector positions from the sector
+    // layout (center = index * sw + floor(sw / 2),
+    // gpuNUFFT_operator_factory.cpp:207-214);
+    // tables from another layout would be gridded to the wrong place.
+    std::vector<uint32_t> c((size_t)p->S * D);
+    HIP_TRY(hipMemcpy(c.data(), p->centers_d, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t sct = 0; sct < p->S && !rc; sct++) {
+      const uint32_t idx[3] = {sct % p->sectors[0], (sct / p->sectors[0]) % p->sectors[1],
+                               sct / (p->sectors[0] * p->sectors[1])};
+      for (int a = 0; a < D; a++)
+        if (c[(size_t)sct * D + a] != idx[a] * p->sw + p->sw / 2) {
+          rc = fail(GNUFFT_ERR_INVALID_ARGUMENT,
+                    "load_precomputed: sector centers differ from the plan's sector layout");
+          break;
+        }
+    }
+  }
+  if (!rc) {
     p->M = M;
     p->g.M = (int)M;
     rc = build_chunks(p, s);
@@ -772,19 +872,16 @@ int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uin
   if (!p || !grid || (!data_sorted && p->M)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "spread_sorted: null");
   if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "spread_sorted: no trajectory set");
   hipStream_t s = (hipStream_t)stream;
-  TIMED(p, GNUFFT_T_ZERO, s, launch_zero(grid, (size_t)n_coils * p->g.Ng * sizeof(float2), s));
-  SpreadArgs a{};
-  a.data = (const float2*)data_sorted;
-  a.coil_stride = p->M;
-  a.crds = p->traj_sorted_d;
-  a.chunks = p->chunks_d;
-  a.n_chunks = p->n_chunks;
-  a.centers = p->centers_d;
-  a.lut = p->lut_d;
-  a.grid = (float2*)grid;
-  a.n_coils = (int)n_coils;
-  a.fold_shift = false;
-  TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, (int)p->chunk, s));
+  const uint32_t cc = coil_batch(p, n_coils);
+  int rc = ensure_scratch(p, cc);
+  if (rc) return rc;
+  for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
+    const uint32_t nc = std::min(cc, n_coils - c0);
+    SpreadArgs a = spread_args(p, (const float2*)data_sorted + (size_t)c0 * p->M,
+                               (float2*)grid + (size_t)c0 * p->g.Ng, nc);
+    a.fold_shift = false;
+    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, (int)p->chunk, s));
+  }
   return GNUFFT_OK;
 }
 
@@ -795,8 +892,7 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   InterpArgs a{};
   a.grid = (const float2*)grid;
   a.crds = p->traj_sorted_d;
-  a.chunks = p->chunks_d;
-  a.n_chunks = p->n_chunks;
+  a.work = p->work;
   a.centers = p->centers_d;
   a.lut = p->lut_d;
   a.out = (float2*)data_sorted;
@@ -819,27 +915,21 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
   hipStream_t s = (hipStream_t)stream;
   const Geom& g = p->g;
   const bool grid_out = stop == GNUFFT_STOP_CONVOLUTION || stop == GNUFFT_STOP_DENSITY_ESTIMATION;
-  const uint32_t cc = grid_out ? n_coils : coil_batch(p, n_coils);
+  const uint32_t cc = coil_batch(p, n_coils);
   if (!grid_out) {
     int rc = ensure_grid(p, cc);
+    if (rc) return rc;
+  }
+  {
+    int rc = ensure_scratch(p, cc);
     if (rc) return rc;
   }
   for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
     const uint32_t nc = std::min(cc, n_coils - c0);
     float2* grid = grid_out ? (float2*)out + (size_t)c0 * g.Ng : p->grid_d;
-    TIMED(p, GNUFFT_T_ZERO, s, launch_zero(grid, (size_t)nc * g.Ng * sizeof(float2), s));
-    SpreadArgs a{};
-    a.data = (const float2*)kdata + (size_t)c0 * p->M;
-    a.coil_stride = p->M;
+    SpreadArgs a = spread_args(p, (const float2*)kdata + (size_t)c0 * p->M, grid, nc);
     a.gather = p->idx_d;
     a.dens = p->dens_sorted_d;
-    a.crds = p->traj_sorted_d;
-    a.chunks = p->chunks_d;
-    a.n_chunks = p->n_chunks;
-    a.centers = p->centers_d;
-    a.lut = p->lut_d;
-    a.grid = grid;
-    a.n_coils = (int)nc;
     a.fold_shift = !grid_out;
     TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, (int)p->chunk, s));
     if (grid_out) continue;
@@ -884,8 +974,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     InterpArgs a{};
     a.grid = p->img_tmp_d;
     a.crds = p->traj_sorted_d;
-    a.chunks = p->chunks_d;
-    a.n_chunks = p->n_chunks;
+    a.work = p->work;
     a.centers = p->centers_d;
     a.lut = p->lut_d;
     a.out = (float2*)out;
@@ -909,8 +998,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     InterpArgs a{};
     a.grid = p->grid_d;
     a.crds = p->traj_sorted_d;
-    a.chunks = p->chunks_d;
-    a.n_chunks = p->n_chunks;
+    a.work = p->work;
     a.centers = p->centers_d;
     a.lut = p->lut_d;
     a.out = (float2*)out + (size_t)c0 * p->M;

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace gnufft {
 
@@ -164,17 +165,48 @@ hipError_t launch_fill_chunks(const uint32_t* sdc, const uint32_t* chunk_off, ui
                               uint32_t chunk, Chunk* chunks, hipStream_t s);
 
 // gridding.hip
+
+// Gridding work list: chunks in sector order, split into 8 contiguous
+// ranges of about equal cost; range x is processed on XCD x (workgroups
+// are dispatched round-robin over the XCDs), so neighbouring sectors share
+// an L2 and no XCD is left with the dense k-space centre alone.
+struct WorkList {
+  const Chunk* chunks;
+  uint32_t n_chunks;
+  uint32_t xcd_begin[9];  // range x = [xcd_begin[x], xcd_begin[x+1])
+  uint32_t max_range;     // longest range: workgroups per XCD
+};
+
+// Split `n` chunks of the given sample counts into the 8 ranges of `w`.
+void partition_work(const uint32_t* counts, uint32_t n, WorkList* w);
+
+// Padded-sector coverage of the grid, per axis a: grid coordinate q along
+// a lies in the padded region of the sectors listed in
+// ent[a][off[a][q] .. off[a][q+1]), each entry (sector index along a) << 8 |
+// (padded index along a). Periodic wrap makes some coordinates appear in
+// several entries of one sector when the grid is small.
+struct Cover {
+  const uint32_t* off[3];
+  const uint32_t* ent[3];
+};
+
+// Host: build the per-axis lists of `g` as one array (offsets of axis a at
+// word base[a], entries at word base[3 + a]).
+void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6]);
+
 struct SpreadArgs {
   const float2* data;      // [coil][...] samples
   long long coil_stride;   // elements between coils of `data`
   const uint32_t* gather;  // NULL: data already in sector order
   const float* dens;       // NULL: no density weighting (sorted order)
   const float* crds;       // sorted SoA [dim][M]
-  const Chunk* chunks;
-  uint32_t n_chunks;
+  WorkList work;
   const uint32_t* centers; // [S][dim]
   const float* lut;
-  float2* grid;            // [coil][Ng]
+  float2* scratch;         // [chunk][coil][padded point] sub-grids (n_chunks*n_coils*PD)
+  const uint32_t* sector_chunks;  // [S+1] first chunk of each sector
+  Cover cover;
+  float2* grid;            // [coil][Ng], every point written (no prior zeroing)
   int n_coils;
   bool fold_shift;         // store at INVERSE-fftshifted positions
 };
@@ -183,8 +215,7 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStrea
 struct InterpArgs {
   const float2* grid;      // [coil][Ng]
   const float* crds;
-  const Chunk* chunks;
-  uint32_t n_chunks;
+  WorkList work;
   const uint32_t* centers;
   const float* lut;
   float2* out;             // [coil][...]
@@ -194,7 +225,6 @@ struct InterpArgs {
   float scale;             // 1 (kernel only) or 1/sqrt(Ng)
   int n_coils;
   bool fold_shift;         // read grid at FORWARD-fftshifted positions
-  bool image_grid;         // grid has image dims (DENSITY_ESTIMATION forward)
 };
 hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s);
 
