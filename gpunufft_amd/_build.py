@@ -95,10 +95,22 @@ def build_python():
     return out
 
 
+def build_cpp_check():
+    """tests/cpp/cpp_api_check: GPU self-check of the C++ API (run by a test)."""
+    src = os.path.join(ROOT, "tests", "cpp", "cpp_api_check.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "cpp_api_check")
+    _run(["g++"] + CXX_FLAGS + ["-o", out, src, "-L", LIB, "-lgpuNUFFT", "-lgpunufft_hip",
+                                "-Wl,-rpath,$ORIGIN/../../gpunufft_amd/lib"])
+    return out
+
+
 def build_all(force=True):
     outs = [build_hip(force=force)]
     outs.append(build_host())
     outs.append(build_python())
+    outs.append(build_cpp_check())
     return [o for o in outs if o]
 
 
