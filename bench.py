@@ -9,12 +9,13 @@ One step = one full adjoint (k-space [coil][sample] in the caller's order in
 HBM -> coil-combined image) plus one full forward (image -> k-space) through
 the C ABI, everything resident in HBM. value = 2*M*C / step time.
 
-roofline: the adjoint spread kernel, algorithmic bytes per launch
-8*M*C + 4*d*M + 4*M (density) + 8*Ng*C, over its average launch time measured
-with HIP events on the launch stream (library stage timer).
+roofline: the adjoint gridding stage (spread_kernel, reduce_chunks_kernel,
+fold_kernel: samples in HBM -> grid in HBM), algorithmic bytes per launch
+8*M*C + 4*d*M + 4*M (density) + 8*Ng*C, over the stage's average time
+measured with HIP events on the launch stream (library stage timer).
 cpu_baseline: the oracle restatement (single thread) running the same
-adjoint + forward on one of the eight coils of the same workload; its output
-doubles as the parity check of the GPU result for that coil.
+adjoint + forward (all 8 coils, sensitivities, density) on the host; its
+outputs double as the full-size parity check of the GPU outputs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -41,8 +42,18 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default=WORKLOAD,
+                    choices=[WORKLOAD, "uneven_batch_2d"],
+                    help="headline 3-D workload, or 64 uneven 2-D operators batched vs "
+                         "sequential plain launches")
+    ap.add_argument("--sector-widths", default="8",
+                    help="uneven_batch_2d: comma-separated sector widths the autotuner "
+                         "chooses from per operator (e.g. 4,8,16)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps (for rocprofv3), no baseline")
+    ap.add_argument("--profile-copy-mb", type=int, default=0,
+                    help="in profile mode, also copy this many MiB device-to-device "
+                         "(known-size reference for HBM traffic counters)")
     return ap.parse_args()
 
 
@@ -69,6 +80,8 @@ def main():
     from gpunufft_amd import workloads as W
 
     torch.cuda.set_device(0)
+    if args.workload == "uneven_batch_2d":
+        return bench_batch(args)
     w = W.config(WORKLOAD)
     traj, img, C, dens = w["traj"], w["img_dims"], w["coils"], w["dens"]
     d, M = traj.shape
@@ -97,8 +110,13 @@ def main():
     if args.profile_steps:
         for _ in range(args.profile_steps):
             step()
+        if args.profile_copy_mb:
+            a = torch.empty(args.profile_copy_mb << 18, dtype=torch.float32, device="cuda")
+            b = torch.empty_like(a)
+            b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_steps": args.profile_steps}))
+        print(json.dumps({"profile_steps": args.profile_steps,
+                          "copy_bytes": args.profile_copy_mb << 20}))
         return
 
     for _ in range(args.warmup):
@@ -125,7 +143,8 @@ def main():
     achieved = spread_bytes / (spread_ms * 1e-3) / 1e9
     traffic = traffic_summary()
     roofline = {
-        "kernel": "spread_kernel<3,4,8> (adjoint gridding)",
+        "kernel": "adjoint gridding stage: spread_kernel<3,4,8> + reduce_chunks_kernel + "
+                  "fold_kernel<3,8>",
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
@@ -140,6 +159,8 @@ def main():
         "metric": "gridded samples/s/GPU adj+fwd at rel-L2<=1e-5 (%HBM roof); p50/p99 CG-SENSE ms",
         "value": round(value, 1),
         "unit": "gridded samples/s",
+        "value_definition": "2*M*C per step / step time (whole adjoint+forward operator incl. "
+                            "FFTs); gridding_only.samples_per_s = 2*M*C/(t_spread+t_interp)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -166,42 +187,134 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        out["cpu_baseline"], out["parity_rel_l2_coil0"] = cpu_baseline(w, plan, kd, torch)
+        out["cpu_baseline"], out["parity_rel_l2"] = cpu_baseline(w, plan, sens, kd, torch)
     print(json.dumps(out))
 
 
-def cpu_baseline(w, plan, kd, torch):
-    """Oracle restatement, one thread, adjoint + forward of coil 0 (no
-    sensitivities) of the same workload; also the parity check of that coil."""
+def cpu_baseline(w, plan, sens, kd, torch):
+    """Oracle restatement, one thread: the same adjoint + forward (all coils,
+    sensitivities, density) on the host. Its outputs are the full-size parity
+    check of the GPU adjoint and forward on the same inputs."""
     import numpy as np
 
     from oracle import oracle as O
 
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
     traj, img = w["traj"], w["img_dims"]
     M = traj.shape[1]
-    op = O.Operator(traj, img, w["kw"], w["sw"], w["osf"], dens=w["dens"], order="stable")
-    y0 = kd[:1]
+    C = kd.shape[0]
+    op = O.Operator(traj, img, w["kw"], w["sw"], w["osf"], dens=w["dens"], sens=sens,
+                    order="stable")
     t0 = time.perf_counter()
-    x_ref = op.adjoint(y0)
+    x_ref = op.adjoint(kd)
     k_ref = op.forward(x_ref)
     t = time.perf_counter() - t0
-    # GPU, same single-coil problem (no sensitivities) for the parity figure
-    plan.set_sens(None, 0)
-    xi = torch.empty(x_ref.shape, dtype=torch.complex64, device="cuda")
-    ko = torch.empty((1, M), dtype=torch.complex64, device="cuda")
-    plan.adjoint(torch.from_numpy(np.ascontiguousarray(y0)).cuda(), xi, 1)
-    plan.forward(xi, ko, 1)
+    x_gpu = torch.empty(x_ref.shape, dtype=torch.complex64, device="cuda")
+    plan.adjoint(torch.from_numpy(np.ascontiguousarray(kd)).cuda(), x_gpu, C)
+    k_gpu = torch.empty((C, M), dtype=torch.complex64, device="cuda")
+    plan.forward(torch.from_numpy(np.ascontiguousarray(x_ref)).cuda(), k_gpu, C)
     torch.cuda.synchronize()
-    err = max(O.rel_l2(xi.cpu().numpy(), x_ref), O.rel_l2(ko.cpu().numpy(), k_ref))
-    base = {
-        "value": round(2.0 * M / t, 1),
+    parity = {"adjoint": O.rel_l2(x_gpu.cpu().numpy(), x_ref),
+              "forward": O.rel_l2(k_gpu.cpu().numpy(), k_ref)}
+    return ({
+        "value": round(2.0 * M * C / t, 1),
         "unit": "gridded samples/s",
         "cores": 1,
         "kind": "port",
-        "sample": "adjoint+forward of coil 0 of 8 (M=%d samples, no sens), %.1f s" % (M, t),
+        "sample": "full workload: adjoint+forward of all %d coils (M=%d, sens, density), "
+                  "oracle restatement, 1 thread, %.1f s" % (C, M, t),
+    }, parity)
+
+
+def bench_batch(args):
+    """64 independent 2-D operators (uneven sizes, spiral trajectories, 4-16
+    coils, sens + density): one step = adjoint + forward of every operator,
+    run (a) as 64 sequential plain per-operator calls and (b) as one batch
+    (one launch per stage). value = batched gridded samples/s."""
+    import numpy as np
+    import torch
+
+    from gpunufft_amd import autotune, capi
+    from gpunufft_amd import workloads as W
+
+    ops = W.uneven_batch_2d()
+    widths = [int(x) for x in args.sector_widths.split(",")]
+    t0 = time.perf_counter()
+    plans, kd, img, kout, iout, sws = [], [], [], [], [], []
+    for j, o in enumerate(ops):
+        C = o["coils"]
+        N = o["img_dims"][0]
+        traj_d = torch.from_numpy(o["traj"]).cuda()
+        dens_d = torch.from_numpy(o["dens"]).cuda()
+        sw = widths[0] if len(widths) == 1 else autotune.best_sector_width(
+            o["img_dims"], o["kw"], o["osf"], traj_d, dens_d, C, widths)
+        sws.append(sw)
+        p = capi.Plan(o["img_dims"], o["kw"], sw, o["osf"])
+        p.set_trajectory(traj_d, dens_d)
+        p.set_sens(torch.from_numpy(o["sens"]).cuda(), C)
+        plans.append(p)
+        rng = np.random.default_rng(o["seed"])
+        M = o["traj"].shape[1]
+        kd.append(torch.from_numpy(W.complex_normal(rng, (C, M))).cuda())
+        img.append(torch.from_numpy(W.complex_normal(rng, (1, N, N))).cuda())
+        kout.append(torch.empty((C, M), dtype=torch.complex64, device="cuda"))
+        iout.append(torch.empty((1, N, N), dtype=torch.complex64, device="cuda"))
+    setup_s = time.perf_counter() - t0
+    coils = [o["coils"] for o in ops]
+    batch = capi.Batch(plans, coils)
+    MC = sum(o["traj"].shape[1] * o["coils"] for o in ops)
+
+    def plain_step():
+        for p, k, i, ko, io, C in zip(plans, kd, img, kout, iout, coils):
+            p.adjoint(k, io, C)
+            p.forward(i, ko, C)
+
+    def batch_step():
+        batch.adjoint(kd, iout)
+        batch.forward(img, kout)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / args.steps
+
+    t_plain = timed(plain_step)
+    ref_a = [x.clone() for x in iout]
+    ref_f = [x.clone() for x in kout]
+    t_batch = timed(batch_step)
+    err = 0.0
+    for a, b_, c, d in zip(iout, ref_a, kout, ref_f):
+        for x, y in ((a, b_), (c, d)):
+            num = torch.linalg.vector_norm((x - y).flatten()).item()
+            den = torch.linalg.vector_norm(y.flatten()).item()
+            err = max(err, num / den if den > 0 else num)
+    out = {
+        "metric": "gridded samples/s adj+fwd, 64 uneven 2-D operators: batched vs plain launches",
+        "value": round(2.0 * MC / t_batch, 1),
+        "unit": "gridded samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded variable-density spirals, complex-normal data, Gaussian coil maps)",
+        "config": {"workload": "uneven_batch_2d: 64 operators, N in {96..384}, spirals 8-48 x "
+                               "1024-4096, 4-16 coils, osf 2, kw 3, sens+dens",
+                   "sample_coil_pairs": MC, "sector_widths": sws},
+        "plain_ms_per_step": round(t_plain * 1e3, 4),
+        "plain_value": round(2.0 * MC / t_plain, 1),
+        "speedup_batched_vs_plain": round(t_plain / t_batch, 3),
+        "batched_vs_plain_rel_l2": err,
+        "setup_s": round(setup_s, 2),
     }
-    return base, err
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

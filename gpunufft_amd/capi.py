@@ -27,7 +27,8 @@ EXPORTS = [
     "gnufft_plan_load_precomputed", "gnufft_plan_set_sens", "gnufft_plan_get_tables",
     "gnufft_plan_get_assigned_sectors", "gnufft_adjoint", "gnufft_forward",
     "gnufft_spread_sorted", "gnufft_interp_sorted", "gnufft_plan_set_timing",
-    "gnufft_plan_get_timing",
+    "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
+    "gnufft_batch_adjoint", "gnufft_batch_forward",
 ]
 
 T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
@@ -112,6 +113,11 @@ def load(path=LIB_PATH):
     L.gnufft_interp_sorted.argtypes = [vp, vp, vp, u32, vp]
     L.gnufft_plan_set_timing.argtypes = [vp, i32]
     L.gnufft_plan_get_timing.argtypes = [vp, ctypes.POINTER(Timing), i32]
+    vpp = ctypes.POINTER(vp)
+    L.gnufft_batch_create.argtypes = [ctypes.POINTER(vp), vpp, u32, u32p, vp]
+    L.gnufft_batch_destroy.argtypes = [vp]
+    L.gnufft_batch_adjoint.argtypes = [vp, vpp, vpp, vp]
+    L.gnufft_batch_forward.argtypes = [vp, vpp, vpp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or i32
     L.gnufft_last_error.restype = ctypes.c_char_p
@@ -256,3 +262,43 @@ class Plan:
         check(load().gnufft_interp_sorted(self._h, _ptr_loc(grid)[0], _ptr_loc(data_sorted)[0],
                                           n_coils, _stream_ptr(stream)))
         return data_sorted
+
+
+class Batch:
+    """Independent plans run together, one launch per stage
+    (gnufft_batch_*). The plans must outlive the batch."""
+
+    def __init__(self, plans, n_coils, stream=None):
+        L = load()
+        self._plans = list(plans)
+        n = len(self._plans)
+        arr = (ctypes.c_void_p * n)(*[p.handle.value for p in self._plans])
+        coils = (ctypes.c_uint32 * n)(*[int(c) for c in n_coils])
+        self._h = ctypes.c_void_p()
+        check(L.gnufft_batch_create(ctypes.byref(self._h), arr, n, coils, _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().gnufft_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ptrs(tensors):
+        return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+    def adjoint(self, kdata, out, stream=None):
+        """kdata[i] (C_i, M_i) and out[i] (1 or C_i, *img_i): device tensors."""
+        check(load().gnufft_batch_adjoint(self._h, self._ptrs(kdata), self._ptrs(out),
+                                          _stream_ptr(stream)))
+        return out
+
+    def forward(self, img, out, stream=None):
+        check(load().gnufft_batch_forward(self._h, self._ptrs(img), self._ptrs(out),
+                                          _stream_ptr(stream)))
+        return out

@@ -189,6 +189,8 @@ struct gnufft_plan_s {
   uint32_t chunk = 1024;
   WorkList work{};
   uint32_t* sector_chunks_d = nullptr;  // [S+1] first chunk of each sector
+  uint2* reduce_d = nullptr;            // (first chunk, count) of multi-chunk sectors
+  uint32_t n_reduce = 0;
   // per-axis padded-sector cover lists of the grid (fold)
   uint32_t* cover_d = nullptr;
   Cover cover{};
@@ -231,7 +233,10 @@ void free_traj(gnufft_plan p) {
   dfree(p->centers_d);
   dfree(p->chunks_d);
   dfree(p->sector_chunks_d);
+  dfree(p->reduce_d);
   dfree(p->scratch_d);
+  p->reduce_d = nullptr;
+  p->n_reduce = 0;
   p->traj_sorted_d = p->dens_sorted_d = nullptr;
   p->idx_d = p->assigned_d = p->sdc_d = p->centers_d = nullptr;
   p->chunks_d = nullptr;
@@ -389,15 +394,11 @@ uint32_t coil_batch(gnufft_plan p, uint32_t n_coils) {
   return (uint32_t)std::min<size_t>(cc, n_coils);
 }
 
-int get_fft(gnufft_plan p, uint32_t batch, FftPlans** out) {
-  auto it = p->ffts.find(batch);
-  if (it != p->ffts.end()) {
-    *out = &it->second;
-    return GNUFFT_OK;
-  }
+// In-place C2C rocFFT plans (both directions) of `batch` grids of g's size.
+int create_fft(const Geom& g, uint32_t batch, FftPlans* out) {
   FftPlans f;
-  size_t lengths[3] = {(size_t)p->g.G[0], (size_t)p->g.G[1], (size_t)p->g.G[2]};
-  const size_t dims = (size_t)p->g.dim;
+  size_t lengths[3] = {(size_t)g.G[0], (size_t)g.G[1], (size_t)g.G[2]};
+  const size_t dims = (size_t)g.dim;
   ROCFFT_TRY(rocfft_plan_create(&f.fwd, rocfft_placement_inplace,
                                 rocfft_transform_type_complex_forward, rocfft_precision_single,
                                 dims, lengths, batch, nullptr));
@@ -411,8 +412,36 @@ int get_fft(gnufft_plan p, uint32_t batch, FftPlans** out) {
   if (f.work_bytes) HIP_TRY(hipMalloc(&f.work, f.work_bytes));
   ROCFFT_TRY(rocfft_execution_info_create(&f.info));
   if (f.work_bytes) ROCFFT_TRY(rocfft_execution_info_set_work_buffer(f.info, f.work, f.work_bytes));
+  *out = f;
+  return GNUFFT_OK;
+}
+
+void destroy_fft(FftPlans& f) {
+  if (f.fwd) rocfft_plan_destroy(f.fwd);
+  if (f.inv) rocfft_plan_destroy(f.inv);
+  if (f.info) rocfft_execution_info_destroy(f.info);
+  if (f.work) (void)hipFree(f.work);
+  f = FftPlans();
+}
+
+int get_fft(gnufft_plan p, uint32_t batch, FftPlans** out) {
+  auto it = p->ffts.find(batch);
+  if (it != p->ffts.end()) {
+    *out = &it->second;
+    return GNUFFT_OK;
+  }
+  FftPlans f;
+  int rc = create_fft(p->g, batch, &f);
+  if (rc) return rc;
   auto ins = p->ffts.emplace(batch, f);
   *out = &ins.first->second;
+  return GNUFFT_OK;
+}
+
+int exec_fft(FftPlans& f, bool inverse, void* buf, hipStream_t s) {
+  ROCFFT_TRY(rocfft_execution_info_set_stream(f.info, s));
+  void* bufs[1] = {buf};
+  ROCFFT_TRY(rocfft_execute(inverse ? f.inv : f.fwd, bufs, nullptr, f.info));
   return GNUFFT_OK;
 }
 
@@ -470,6 +499,20 @@ int build_chunks(gnufft_plan p, hipStream_t s) {
   for (uint32_t i = 0; i < total; i++) counts[i] = h[i].end - h[i].begin;
   p->work.chunks = p->chunks_d;
   partition_work(counts.data(), total, &p->work);
+  // sectors split over several chunks: their sub-grids are summed before the fold
+  std::vector<uint2> tasks;
+  for (uint32_t i = 0; i < total;) {
+    uint32_t j = i + 1;
+    while (j < total && h[j].sector == h[i].sector) j++;
+    if (j - i > 1) tasks.push_back(make_uint2(i, j - i));
+    i = j;
+  }
+  p->n_reduce = (uint32_t)tasks.size();
+  if (!tasks.empty()) {
+    int rc = copy_in((void**)&p->reduce_d, tasks.data(), tasks.size() * sizeof(uint2), GNUFFT_HOST, s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   return GNUFFT_OK;
 }
 
@@ -487,6 +530,8 @@ SpreadArgs spread_args(gnufft_plan p, const float2* data, float2* grid, uint32_t
   a.lut = p->lut_d;
   a.scratch = p->scratch_d;
   a.sector_chunks = p->sector_chunks_d;
+  a.reduce_tasks = p->reduce_d;
+  a.n_reduce = p->n_reduce;
   a.cover = p->cover;
   a.grid = grid;
   a.n_coils = (int)n_coils;
@@ -634,7 +679,7 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   if (rc == GNUFFT_OK) {
     std::vector<uint32_t> words;
     size_t base[6];
-    build_cover(g, &words, base);
+    build_cover(g, &words, base, p->cover.max_len);
     rc = copy_in((void**)&p->cover_d, words.data(), words.size() * sizeof(uint32_t), GNUFFT_HOST, s);
     if (rc == GNUFFT_OK) {
       for (int a = 0; a < 3; a++) {
@@ -1010,6 +1055,292 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.fold_shift = true;
     TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->chunk, s));
   }
+  return GNUFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent plans
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct gnufft_batch_s {
+  int dim = 2, nx = 4;
+  std::vector<gnufft_plan> plans;  // in batch order (grouped by grid size)
+  std::vector<uint32_t> coils;
+  std::vector<uint32_t> caller_index;  // batch position -> caller's plan index
+  std::vector<BatchOp> ops_h;
+  BatchOp* ops_d = nullptr;
+  uint4* items_d = nullptr;
+  BatchList spread{}, interp{}, fold{}, reduce{}, post{}, pre{};
+  size_t spread_lds = 0, interp_lds = 0;
+  int fold_threads = 64;
+  uint32_t max_kstride = 0;
+  float2* grid_d = nullptr;
+  float2* scratch_d = nullptr;
+  struct Group {
+    uint32_t first, count;  // plans [first, first + count) in batch order
+    size_t offset;          // grid elements before the group
+    FftPlans fft;
+  };
+  std::vector<Group> groups;
+  hipEvent_t uploaded = nullptr;  // last descriptor upload (ops_h may be reused after it)
+};
+
+namespace {
+
+void free_batch(gnufft_batch b) {
+  if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+  dfree(b->ops_d);
+  dfree(b->items_d);
+  dfree(b->grid_d);
+  dfree(b->scratch_d);
+  for (auto& gr : b->groups) destroy_fft(gr.fft);
+  delete b;
+}
+
+// Wait until the previous descriptor upload has read ops_h.
+int batch_wait_upload(gnufft_batch b) {
+  if (b->uploaded) HIP_TRY(hipEventSynchronize(b->uploaded));
+  return GNUFFT_OK;
+}
+
+int batch_upload_ops(gnufft_batch b, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(b->ops_d, b->ops_h.data(), b->ops_h.size() * sizeof(BatchOp),
+                         hipMemcpyHostToDevice, s));
+  if (!b->uploaded) HIP_TRY(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(b->uploaded, s));
+  return GNUFFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t n_plans,
+                        const uint32_t* n_coils, void* stream) {
+  if (!batch || !plans || !n_coils || n_plans == 0)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_create: null argument or empty batch");
+  *batch = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  const Geom& g0 = plans[0]->g;
+  const int nx = g0.kw + 1 <= 4 ? 4 : 8;
+  for (uint32_t i = 0; i < n_plans; i++) {
+    const gnufft_plan p = plans[i];
+    if (!p || !p->sdc_d) return fail(GNUFFT_ERR_STATE, "batch_create: plan without trajectory");
+    if (p->g.dim != g0.dim || (p->g.kw + 1 <= 4 ? 4 : 8) != nx)
+      return fail(GNUFFT_ERR_UNSUPPORTED,
+                  "batch_create: plans must share the dimension and the kernel-width class");
+    if (n_coils[i] == 0) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_create: zero coils");
+    if (p->n_sens > 0 && p->n_sens != n_coils[i])
+      return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_create: coils differ from sensitivity coils");
+  }
+  gnufft_batch b = new (std::nothrow) gnufft_batch_s();
+  if (!b) return fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: host allocation");
+  b->dim = g0.dim;
+  b->nx = nx;
+  // batch order: grouped by grid size (one rocFFT plan per group)
+  std::vector<uint32_t> order(n_plans);
+  for (uint32_t i = 0; i < n_plans; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+    const Geom &ga = plans[a]->g, &gc = plans[c]->g;
+    if (ga.G[2] != gc.G[2]) return ga.G[2] < gc.G[2];
+    if (ga.G[1] != gc.G[1]) return ga.G[1] < gc.G[1];
+    return ga.G[0] < gc.G[0];
+  });
+  const size_t half = lds_budget() / 2 - 1024;
+  size_t grid_total = 0, scratch_total = 0;
+  std::vector<size_t> grid_off(n_plans), scratch_off(n_plans);
+  for (uint32_t k = 0; k < n_plans; k++) {
+    const gnufft_plan p = plans[order[k]];
+    const uint32_t C = n_coils[order[k]];
+    b->plans.push_back(p);
+    b->coils.push_back(C);
+    b->caller_index.push_back(order[k]);
+    grid_off[k] = grid_total;
+    grid_total += (size_t)C * p->g.Ng;
+    scratch_off[k] = scratch_total;
+    scratch_total += (size_t)p->n_chunks * C * padded_points(p->g);
+    if (k == 0 || p->g.G[0] != plans[order[k - 1]]->g.G[0] || p->g.G[1] != plans[order[k - 1]]->g.G[1] ||
+        p->g.G[2] != plans[order[k - 1]]->g.G[2])
+      b->groups.push_back({k, 0, grid_off[k], FftPlans()});
+    b->groups.back().count++;
+  }
+  int rc = GNUFFT_OK;
+  if (hipMalloc(&b->grid_d, std::max<size_t>(grid_total, 1) * sizeof(float2)) != hipSuccess ||
+      hipMalloc(&b->scratch_d, std::max<size_t>(scratch_total, 1) * sizeof(float2)) != hipSuccess) {
+    free_batch(b);
+    return fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: grid / scratch allocation");
+  }
+  // descriptors and item lists
+  std::vector<uint4> sp, rd, fo, po, pr;
+  std::vector<uint32_t> sp_cost, fo_cost;
+  for (uint32_t k = 0; k < n_plans && !rc; k++) {
+    const gnufft_plan p = b->plans[k];
+    const uint32_t C = b->coils[k];
+    const uint32_t cbs = (C + kBatchCB - 1) / kBatchCB;
+    BatchOp op{};
+    op.g = p->g;
+    op.s = spread_args(p, nullptr, b->grid_d + grid_off[k], C);
+    op.s.scratch = b->scratch_d + scratch_off[k];
+    op.s.gather = p->idx_d;
+    op.s.dens = p->dens_sorted_d;
+    op.s.fold_shift = true;
+    op.s.n_reduce = 0;
+    op.i.grid = b->grid_d + grid_off[k];
+    op.i.crds = p->traj_sorted_d;
+    op.i.work = p->work;
+    op.i.centers = p->centers_d;
+    op.i.lut = p->lut_d;
+    op.i.coil_stride = p->M;
+    op.i.scatter = p->idx_d;
+    op.i.dens = p->dens_sorted_d;
+    op.i.scale = p->g.fft_scale;
+    op.i.n_coils = (int)C;
+    op.i.fold_shift = true;
+    op.post = PostArgs{b->grid_d + grid_off[k], (int)C, p->deapo_d,
+                       p->n_sens ? p->sens_d : nullptr, nullptr, GNUFFT_STOP_DEAPODIZATION};
+    op.pre = PreArgs{nullptr, p->n_sens ? 0 : 1, p->n_sens ? p->sens_d : nullptr, p->deapo_d,
+                     b->grid_d + grid_off[k], (int)C};
+    op.R_spread = spread_round(p->g, kBatchCB, (int)p->chunk, half);
+    op.R_interp = interp_round(p->g, kBatchCB, (int)p->chunk, half);
+    if (op.R_spread <= 0 || op.R_interp <= 0) {
+      rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: padded sector does not fit in LDS");
+      break;
+    }
+    b->spread_lds = std::max(b->spread_lds, spread_lds_bytes(p->g, kBatchCB, op.R_spread));
+    b->interp_lds = std::max(b->interp_lds, interp_lds_bytes(p->g, kBatchCB, op.R_interp));
+    const int ft = fold_threads(p->g, p->cover);
+    if (ft == 0) {
+      rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: too many fold candidates");
+      break;
+    }
+    b->fold_threads = std::max(b->fold_threads, ft);
+    b->max_kstride = std::max<uint32_t>(b->max_kstride, C * (uint32_t)padded_points(p->g));
+    b->ops_h.push_back(op);
+    // chunk list (host copy) for the gridding items and the reduce tasks
+    std::vector<Chunk> ch(p->n_chunks);
+    if (p->n_chunks &&
+        hipMemcpy(ch.data(), p->chunks_d, ch.size() * sizeof(Chunk), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(GNUFFT_ERR_DEVICE, "batch_create: chunk list copy");
+      break;
+    }
+    for (uint32_t ci = 0; ci < p->n_chunks; ci++)
+      for (uint32_t cb = 0; cb < cbs; cb++) {
+        sp.push_back(make_uint4(k, ci, cb, 0));
+        sp_cost.push_back((ch[ci].end - ch[ci].begin) + 300u);
+      }
+    for (uint32_t i = 0; i < p->n_chunks;) {
+      uint32_t j = i + 1;
+      while (j < p->n_chunks && ch[j].sector == ch[i].sector) j++;
+      if (j - i > 1) rd.push_back(make_uint4(k, i, j - i, 0));
+      i = j;
+    }
+    for (uint32_t sct = 0; sct < p->S; sct++)
+      for (uint32_t cb = 0; cb < cbs; cb++) {
+        fo.push_back(make_uint4(k, sct, cb, 0));
+        fo_cost.push_back(1u);
+      }
+    for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ni; blk++) po.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
+    for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ng; blk++) pr.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
+  }
+  if (!rc) {
+    std::vector<uint4> all;
+    all.insert(all.end(), sp.begin(), sp.end());
+    all.insert(all.end(), rd.begin(), rd.end());
+    all.insert(all.end(), fo.begin(), fo.end());
+    all.insert(all.end(), po.begin(), po.end());
+    all.insert(all.end(), pr.begin(), pr.end());
+    if (hipMalloc(&b->ops_d, b->ops_h.size() * sizeof(BatchOp)) != hipSuccess ||
+        hipMalloc(&b->items_d, std::max<size_t>(all.size(), 1) * sizeof(uint4)) != hipSuccess) {
+      rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: descriptor allocation");
+    } else if (hipMemcpyAsync(b->items_d, all.data(), all.size() * sizeof(uint4),
+                              hipMemcpyHostToDevice, s) != hipSuccess) {
+      rc = fail(GNUFFT_ERR_DEVICE, "batch_create: item upload");
+    }
+  }
+  if (!rc) {
+    size_t at = 0;
+    auto list = [&](BatchList& l, const std::vector<uint4>& v, const std::vector<uint32_t>* cost) {
+      l.items = b->items_d + at;
+      if (cost) {
+        partition_items(cost->data(), (uint32_t)v.size(), &l);
+      } else {
+        l.n = (uint32_t)v.size();
+      }
+      at += v.size();
+    };
+    list(b->spread, sp, &sp_cost);
+    b->interp = b->spread;
+    list(b->reduce, rd, nullptr);
+    list(b->fold, fo, &fo_cost);
+    list(b->post, po, nullptr);
+    list(b->pre, pr, nullptr);
+    for (auto& gr : b->groups) {
+      uint32_t coils = 0;
+      for (uint32_t k = gr.first; k < gr.first + gr.count; k++) coils += b->coils[k];
+      rc = create_fft(b->plans[gr.first]->g, coils, &gr.fft);
+      if (rc) break;
+    }
+  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GNUFFT_ERR_DEVICE, "batch_create: sync");
+  if (rc) {
+    free_batch(b);
+    return rc;
+  }
+  *batch = b;
+  return GNUFFT_OK;
+}
+
+int gnufft_batch_destroy(gnufft_batch batch) {
+  if (!batch) return GNUFFT_OK;
+  (void)hipDeviceSynchronize();
+  free_batch(batch);
+  return GNUFFT_OK;
+}
+
+int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* out, void* stream) {
+  if (!b || !kdata || !out) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_adjoint: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc0 = batch_wait_upload(b);
+  if (rc0) return rc0;
+  for (size_t k = 0; k < b->ops_h.size(); k++) {
+    const uint32_t ci = b->caller_index[k];
+    if (!kdata[ci] || !out[ci]) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_adjoint: null buffer");
+    b->ops_h[k].s.data = (const float2*)kdata[ci];
+    b->ops_h[k].post.out = (float2*)out[ci];
+  }
+  int rc = batch_upload_ops(b, s);
+  if (rc) return rc;
+  HIP_TRY(launch_spread_batch(b->dim, b->nx, b->ops_d, b->spread, b->spread_lds, s));
+  HIP_TRY(launch_reduce_batch(b->ops_d, b->reduce, b->max_kstride, s));
+  HIP_TRY(launch_fold_batch(b->dim, b->ops_d, b->fold, b->fold_threads, s));
+  for (auto& gr : b->groups) {
+    rc = exec_fft(gr.fft, true, b->grid_d + gr.offset, s);
+    if (rc) return rc;
+  }
+  HIP_TRY(launch_post_batch(b->ops_d, b->post, s));
+  return GNUFFT_OK;
+}
+
+int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* out, void* stream) {
+  if (!b || !img || !out) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_forward: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc0 = batch_wait_upload(b);
+  if (rc0) return rc0;
+  for (size_t k = 0; k < b->ops_h.size(); k++) {
+    const uint32_t ci = b->caller_index[k];
+    if (!img[ci] || !out[ci]) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_forward: null buffer");
+    b->ops_h[k].pre.img = (const float2*)img[ci];
+    b->ops_h[k].i.out = (float2*)out[ci];
+  }
+  int rc = batch_upload_ops(b, s);
+  if (rc) return rc;
+  HIP_TRY(launch_pre_batch(b->ops_d, b->pre, s));
+  for (auto& gr : b->groups) {
+    rc = exec_fft(gr.fft, false, b->grid_d + gr.offset, s);
+    if (rc) return rc;
+  }
+  HIP_TRY(launch_interp_batch(b->dim, b->nx, b->ops_d, b->interp, b->interp_lds, s));
   return GNUFFT_OK;
 }
 

@@ -180,19 +180,19 @@ struct WorkList {
 // Split `n` chunks of the given sample counts into the 8 ranges of `w`.
 void partition_work(const uint32_t* counts, uint32_t n, WorkList* w);
 
-// Padded-sector coverage of the grid, per axis a: grid coordinate q along
-// a lies in the padded region of the sectors listed in
-// ent[a][off[a][q] .. off[a][q+1]), each entry (sector index along a) << 8 |
-// (padded index along a). Periodic wrap makes some coordinates appear in
-// several entries of one sector when the grid is small.
+// Padded-sector coverage, per axis a: the home interval
+// [s*sw, min(s*sw + sw, G)) of sector index s along a meets (with periodic
+// wrap) the padded regions of the sector indices
+// ent[a][off[a][s] .. off[a][s+1]).
 struct Cover {
   const uint32_t* off[3];
   const uint32_t* ent[3];
+  int max_len[3];  // longest list per axis
 };
 
 // Host: build the per-axis lists of `g` as one array (offsets of axis a at
-// word base[a], entries at word base[3 + a]).
-void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6]);
+// word base[a], entries at word base[3 + a]); max_len[a] = longest list.
+void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3]);
 
 struct SpreadArgs {
   const float2* data;      // [coil][...] samples
@@ -205,6 +205,8 @@ struct SpreadArgs {
   const float* lut;
   float2* scratch;         // [chunk][coil][padded point] sub-grids (n_chunks*n_coils*PD)
   const uint32_t* sector_chunks;  // [S+1] first chunk of each sector
+  const uint2* reduce_tasks;      // (first chunk, count) of each multi-chunk sector
+  uint32_t n_reduce;
   Cover cover;
   float2* grid;            // [coil][Ng], every point written (no prior zeroing)
   int n_coils;
@@ -239,5 +241,70 @@ hipError_t launch_forward_pre(const Geom& g, const float2* img, bool img_per_coi
 hipError_t launch_image_prep(const Geom& g, const float2* img, bool img_per_coil,
                              const float2* sens, const float* deapo, float2* out, int n_coils,
                              hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batches of independent operators: every stage of every operator runs in
+// one launch per stage. Workgroups take items (operator, index, coil block)
+// from a flat list split into 8 XCD ranges of equal cost, and read their
+// operator's geometry and arguments from a descriptor in device memory.
+// ---------------------------------------------------------------------------
+struct PostArgs {  // adjoint tail (crop, scale, deapo, coil combine)
+  const float2* grid;
+  int n_coils;
+  const float* deapo;
+  const float2* sens;  // NULL: per-coil output
+  float2* out;
+  int stop;
+};
+
+struct PreArgs {  // forward head (sens, deapo, pad, shift)
+  const float2* img;
+  int img_per_coil;
+  const float2* sens;
+  const float* deapo;
+  float2* grid;
+  int n_coils;
+};
+
+struct BatchOp {
+  Geom g;
+  SpreadArgs s;
+  InterpArgs i;
+  PostArgs post;
+  PreArgs pre;
+  int R_spread;
+  int R_interp;
+};
+
+struct BatchList {
+  const uint4* items;  // (operator, index, coil block, unused)
+  uint32_t n;
+  uint32_t xcd_begin[9];
+  uint32_t max_range;
+};
+
+// Split items of the given costs into the 8 XCD ranges of `l` (items stay in order).
+void partition_items(const uint32_t* costs, uint32_t n, BatchList* l);
+
+// gridding.hip (CB: coils per workgroup, the same for every operator)
+constexpr int kBatchCB = 4;
+hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
+                               size_t lds, hipStream_t s);
+hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
+                               hipStream_t s);
+hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, int threads,
+                             hipStream_t s);
+hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
+                               size_t lds, hipStream_t s);
+size_t spread_lds_bytes(const Geom& g, int cb, int R);
+size_t interp_lds_bytes(const Geom& g, int cb, int R);
+int spread_round(const Geom& g, int cb, int chunk, size_t limit);
+int interp_round(const Geom& g, int cb, int chunk, size_t limit);
+size_t lds_budget();
+int fold_threads(const Geom& g, const Cover& cover);
+
+// prepost.hip: items (operator, block of 256 pixels / grid points, 0, 0)
+hipError_t launch_post_batch(const BatchOp* ops, const BatchList& l, hipStream_t s);
+hipError_t launch_pre_batch(const BatchOp* ops, const BatchList& l, hipStream_t s);
 
 }  // namespace gnufft

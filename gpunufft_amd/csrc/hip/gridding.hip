@@ -27,6 +27,8 @@
 // take samples that lie far apart in the round, so their taps rarely hit
 // the same LDS words.
 
+#include <algorithm>
+
 #include "gnufft_internal.hpp"
 
 namespace gnufft {
@@ -44,27 +46,48 @@ __device__ inline uint32_t work_chunk(const WorkList& w, uint32_t bid) {
 // Workgroups along x for a work list.
 static inline uint32_t work_blocks(const WorkList& w) { return 8u * (w.max_range ? w.max_range : 1u); }
 
-void partition_work(const uint32_t* counts, uint32_t n, WorkList* w) {
-  // Cost of a chunk: its samples plus a fixed share for staging and
-  // flushing the padded sub-grid (about 300 samples' worth on gfx950).
-  constexpr double kChunkCost = 300.0;
+// Split n items of the given costs into 8 contiguous ranges of about equal
+// total cost: begin[x] .. begin[x+1] is range x.
+static void split_ranges(const uint32_t* costs, uint32_t n, uint32_t begin[9], uint32_t* max_range) {
   double total = 0.0;
-  for (uint32_t i = 0; i < n; i++) total += (double)counts[i] + kChunkCost;
+  for (uint32_t i = 0; i < n; i++) total += (double)costs[i];
   uint32_t x = 0;
   double acc = 0.0;
-  w->xcd_begin[0] = 0;
+  begin[0] = 0;
   for (uint32_t i = 0; i < n && x < 7; i++) {
-    acc += (double)counts[i] + kChunkCost;
-    while (x < 7 && acc >= total * (double)(x + 1) / 8.0) w->xcd_begin[++x] = i + 1;
+    acc += (double)costs[i];
+    while (x < 7 && acc >= total * (double)(x + 1) / 8.0) begin[++x] = i + 1;
   }
-  while (x < 8) w->xcd_begin[++x] = n;
-  w->n_chunks = n;
-  w->max_range = 0;
-  for (int k = 0; k < 8; k++) {
-    const uint32_t len = w->xcd_begin[k + 1] - w->xcd_begin[k];
-    if (len > w->max_range) w->max_range = len;
-  }
+  while (x < 8) begin[++x] = n;
+  *max_range = 0;
+  for (int k = 0; k < 8; k++) *max_range = std::max(*max_range, begin[k + 1] - begin[k]);
 }
+
+void partition_work(const uint32_t* counts, uint32_t n, WorkList* w) {
+  // Cost of a chunk: its samples plus a fixed share for staging and
+  // storing the padded sub-grid (about 300 samples' worth on gfx950).
+  std::vector<uint32_t> cost(n);
+  for (uint32_t i = 0; i < n; i++) cost[i] = counts[i] + 300u;
+  split_ranges(cost.data(), n, w->xcd_begin, &w->max_range);
+  w->n_chunks = n;
+}
+
+void partition_items(const uint32_t* costs, uint32_t n, BatchList* l) {
+  split_ranges(costs, n, l->xcd_begin, &l->max_range);
+  l->n = n;
+}
+
+// Item of workgroup bid in a batch list (same XCD mapping as work_chunk);
+// false past the end of its XCD's range.
+__device__ inline bool batch_item(const BatchList& l, uint32_t bid, uint4* it) {
+  const uint32_t x = bid & 7u;
+  const uint32_t i = l.xcd_begin[x] + (bid >> 3);
+  if (i >= l.xcd_begin[x + 1]) return false;
+  *it = l.items[i];
+  return true;
+}
+
+static inline uint32_t batch_blocks(const BatchList& l) { return 8u * (l.max_range ? l.max_range : 1u); }
 
 template <int DIM>
 __device__ inline void load_center(const uint32_t* centers, uint32_t sector, int* c) {
@@ -179,9 +202,10 @@ __device__ inline void load_coord(const float* crds, int M, uint32_t s, float* p
 }
 
 // Shared-memory layout of the spread (offsets in bytes from the dynamic
-// LDS base; identical on host and device).
+// LDS base; identical on host and device). The sub-grid holds one 64-bit
+// fixed-point (re, im) pair per (padded point, coil).
 struct SpreadLayout {
-  int kpad, kpw, plane, sub_n;
+  int kpad, kpw, sub_n;
   size_t kp_off, sub_off, rec_off, sv_off, bytes;
   __host__ __device__ SpreadLayout(const Geom& g, int nx, int cb, int R) {
     const int P = g.P;
@@ -189,11 +213,10 @@ struct SpreadLayout {
     const int rw = ((1 + 3 * nx) + 3) & ~3;
     kpad = (g.K + 3) & ~3;
     kpw = (3 * (P + nx) + 3) & ~3;
-    plane = PD * cb;
-    sub_n = 2 * plane + nx * cb;  // + slack for the last row's window overrun
+    sub_n = PD * cb + nx * cb;  // + slack for the last row's window overrun
     kp_off = (size_t)kpad * 4;
     sub_off = kp_off + (size_t)kpw * 4;
-    rec_off = sub_off + (size_t)sub_n * 8;
+    rec_off = sub_off + (size_t)((sub_n + 1) & ~1) * 8;
     sv_off = rec_off + (size_t)R * rw * 4;
     bytes = sv_off + (size_t)R * cb * 8;
   }
@@ -218,20 +241,45 @@ struct InterpLayout {
 
 // ---------------------------------------------------------------------------
 // Spread
+//
+// The sub-grid accumulates in 64-bit fixed point: each (point, coil) word is
+// im * 2^32 + re as one two's-complement integer, re and im each a 32-bit
+// multiple of 1/scale, with scale = 2^30 / L1 and L1 the chunk's sum over
+// samples of max(|re|, |im|) (density-weighted) for that coil. Every kernel
+// weight is at most 1, so no partial sum of either component can reach 2^31
+// and the packed sum never carries wrongly between the halves. One
+// ds_add_u64 per tap replaces two float atomics; integer sums are exact and
+// order-independent, so the result is bit-reproducible. The quantisation
+// step is L1 / 2^30 per term (relative L2 error of a chunk's sub-grid about
+// 5e-10 * samples in the chunk).
 // ---------------------------------------------------------------------------
+constexpr double kRoundMagic = 6755399441055744.0;  // 1.5 * 2^52: x + magic rounds x to an integer
+constexpr double kFixedOne = 1073741824.0;          // 2^30
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// One workgroup's share of the spread: chunk ci of the work list, coils
+// [coil_block * CB, coil_block * CB + CB).
 template <int DIM, int NX, int CB>
-__global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, int R) {
+__device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t ci, int coil_block,
+                                   int R) {
   constexpr int NT = kThreads;
   constexpr int NW = NT / 64;
   constexpr int SPW = 64 / CB;  // sample slots per wave; one lane per (slot, coil)
   constexpr int RW = Rec<NX>::W;
 
   extern __shared__ float4 smem4[];
+  __shared__ float l1_part[NW][CB];
+  __shared__ double scale_s[CB];
   char* smem = reinterpret_cast<char*>(smem4);
   const SpreadLayout L(g, NX, CB, R);
   float* lut = reinterpret_cast<float*>(smem);
   float* kp = reinterpret_cast<float*>(smem + L.kp_off);
-  double* sub = reinterpret_cast<double*>(smem + L.sub_off);
+  unsigned long long* sub = reinterpret_cast<unsigned long long*>(smem + L.sub_off);
   float* rec = reinterpret_cast<float*>(smem + L.rec_off);
   float2* sv = reinterpret_cast<float2*>(smem + L.sv_off);
   const int P = g.P;
@@ -239,46 +287,68 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
   const int KPS = P + NX;
   const int PD = DIM == 3 ? P * P * P : P * P;
 
-  const uint32_t ci = work_chunk(a.work, blockIdx.x);
-  if (ci == ~0u) return;
   const Chunk ch = a.work.chunks[ci];
   const int n = (int)(ch.end - ch.begin);
-  const int coil0 = blockIdx.y * CB;
+  const int coil0 = coil_block * CB;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   int c3[3];
   load_center<DIM>(a.centers, ch.sector, c3);
 
   for (int i = tid; i < g.K; i += NT) lut[i] = a.lut[i];
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
+  for (int i = tid; i < L.sub_n; i += NT) sub[i] = 0ull;
+
+  // per-coil fixed-point scale from the chunk's L1 bound
   {
-    float4* z4 = reinterpret_cast<float4*>(sub);
-    for (int i = tid; i < L.sub_n / 2; i += NT) z4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float l1[CB];
+#pragma unroll
+    for (int cc = 0; cc < CB; cc++) l1[cc] = 0.0f;
+    for (int i = tid; i < n; i += NT) {
+      const uint32_t s = ch.begin + i;
+      const uint32_t src = a.gather ? a.gather[s] : s;
+      const float sq = a.dens ? sqrtf(a.dens[s]) : 1.0f;
+#pragma unroll
+      for (int cc = 0; cc < CB; cc++)
+        if (coil0 + cc < a.n_coils) {
+          const float2 v = a.data[(size_t)(coil0 + cc) * a.coil_stride + src];
+          l1[cc] += fmaxf(fabsf(v.x), fabsf(v.y)) * sq;
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < CB; cc++) {
+      const float t = wave_sum(l1[cc]);
+      if (lane == 0) l1_part[wave][cc] = t;
+    }
+    __syncthreads();
+    if (tid < CB) {
+      double t = 0.0;
+      for (int w = 0; w < NW; w++) t += (double)l1_part[w][tid];
+      scale_s[tid] = t > 0.0 ? kFixedOne / t : 0.0;
+    }
   }
 
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
   const int slot = lane / CB;
   const int c = lane % CB;
   const bool coil_ok = coil0 + c < a.n_coils;
 
   for (int r0 = 0; r0 < n; r0 += R) {
     const int rn = min(R, n - r0);
-    __syncthreads();  // tables ready / previous round consumed
+    __syncthreads();  // tables and scales ready / previous round consumed
     for (int i = tid; i < rn; i += NT) {
       const uint32_t s = ch.begin + r0 + i;
       float p[3];
       load_coord<DIM>(a.crds, g.M, s, p);
       const uint32_t src = a.gather ? a.gather[s] : s;
-      const float sq = a.dens ? sqrtf(a.dens[s]) : 1.0f;
+      const double sq = a.dens ? (double)sqrtf(a.dens[s]) : 1.0;
 #pragma unroll
       for (int cc = 0; cc < CB; cc++) {
         float2 v = make_float2(0.0f, 0.0f);
         if (coil0 + cc < a.n_coils) {
           v = a.data[(size_t)(coil0 + cc) * a.coil_stride + src];
-          if (a.dens) {
-            v.x = v.x * sq;
-            v.y = v.y * sq;
-          }
+          const double k = sq * scale_s[cc];
+          v = make_float2((float)(v.x * k), (float)(v.y * k));
         }
         sv[i * CB + cc] = v;
       }
@@ -313,14 +383,16 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
           if (ty >= ny) break;
           const int iy = min(by + ty, Pmax);
           const float wzy = DIM == 3 ? wz * w[1 + NX + ty] : w[1 + NX + ty];
-          double* re = sub + ((iz * P + iy) * P + bx) * CB + c;
-          double* im = re + L.plane;
+          unsigned long long* row = sub + ((iz * P + iy) * P + bx) * CB + c;
 #pragma unroll
           for (int tx = 0; tx < NX; tx++) {
             if (tx >= nx) break;
             const double wt = (double)(wzy * w[1 + tx]);
-            atomicAdd(re + tx * CB, wt * vx);
-            atomicAdd(im + tx * CB, wt * vy);
+            // round(wt * v) in the low word of (wt * v + magic)
+            const uint32_t re = (uint32_t)__double_as_longlong(fma(wt, vx, kRoundMagic));
+            const uint32_t im = (uint32_t)__double_as_longlong(fma(wt, vy, kRoundMagic));
+            const uint32_t hi = im + (uint32_t)((int32_t)re >> 31);  // borrow of a negative re
+            atomicAdd(row + tx * CB, ((unsigned long long)hi << 32) | re);
           }
         }
       }
@@ -335,8 +407,20 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
     const int pt = e / CB;
     const int cc = e % CB;
     if (coil0 + cc >= a.n_coils) continue;
-    dst[(size_t)cc * PD + pt] = make_float2((float)sub[e], (float)sub[L.plane + e]);
+    const unsigned long long u = sub[e];
+    const int32_t re = (int32_t)(uint32_t)u;
+    const int32_t im = (int32_t)(uint32_t)(u >> 32) - (re >> 31);
+    const double sc = scale_s[cc];
+    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+    dst[(size_t)cc * PD + pt] = make_float2((float)(re * inv), (float)(im * inv));
   }
+}
+
+template <int DIM, int NX, int CB>
+__global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, int R) {
+  const uint32_t ci = work_chunk(a.work, blockIdx.x);
+  if (ci == ~0u) return;
+  spread_body<DIM, NX, CB>(g, a, ci, blockIdx.y, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,93 +428,211 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
 // over each such sector's chunks of the sub-grid value at q's padded index.
 // One workgroup per (home sector, coil); the home sector of q is q / sw per
 // axis, so every grid point is written exactly once, in a fixed summation
-// order (deterministic). Periodic wrap and the optional INVERSE fftshift of
-// the store (std_gpuNUFFT_kernels.cu:629-654) are in the cover lists and
-// the store index.
+// order (deterministic). The loops over candidate sectors and their chunks
+// are uniform across the workgroup; each thread tests its own points
+// arithmetically, so the scratch reads carry no dependent-load chains.
+// Periodic wrap and the optional INVERSE fftshift of the store
+// (std_gpuNUFFT_kernels.cu:629-654) are folded into the indices.
 // ---------------------------------------------------------------------------
-template <int DIM>
-__global__ void __launch_bounds__(256) fold_kernel(Geom g, SpreadArgs a) {
+constexpr int kFoldThreads = 256;
+constexpr int kFoldPts = 2;      // home points per thread per pass
+constexpr int kFoldCands = 256;  // candidate sectors per home sector (host-checked)
+
+// Sum every chunk sub-grid of a multi-chunk sector into the slot of its
+// first chunk, in chunk order (deterministic); the fold then reads one
+// sub-grid per sector. Task (first chunk, chunk count) per blockIdx.y.
+__device__ inline void reduce_body(float2* scratch, size_t kstride, uint32_t k0, uint32_t cnt,
+                                   size_t e) {
+  if (e >= kstride) return;
+  float2* base = scratch + (size_t)k0 * kstride + e;
+  float ax = base[0].x, ay = base[0].y;
+#pragma unroll 8
+  for (uint32_t k = 1; k < cnt; k++) {
+    const float2 v = base[(size_t)k * kstride];
+    ax += v.x;
+    ay += v.y;
+  }
+  base[0] = make_float2(ax, ay);
+}
+
+__global__ void __launch_bounds__(256) reduce_chunks_kernel(const uint2* __restrict__ tasks,
+                                                            float2* __restrict__ scratch,
+                                                            size_t kstride) {
+  const uint2 t = tasks[blockIdx.y];
+  reduce_body(scratch, kstride, t.x, t.y, (size_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <int DIM, int CB>
+__device__ inline void fold_body(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+  // item per non-empty candidate sector: padded origin per axis and the
+  // chunk holding the sector's (reduced) sub-grid; rep: packed recurrence
+  // offsets (P > G only)
+  __shared__ int4 items[kFoldCands];
+  __shared__ uint32_t reps_of[kFoldCands];
+  __shared__ int slot[kFoldCands + 1];
   const int P = g.P;
   const int PD = DIM == 3 ? P * P * P : P * P;
   const int sw = g.sw;
-  const uint32_t s = blockIdx.x;
-  const int c = blockIdx.y;
+  const int coil0 = coil_block * CB;
   const int sx = (int)(s % (uint32_t)g.S[0]);
   const uint32_t r = s / (uint32_t)g.S[0];
   const int sy = (int)(r % (uint32_t)g.S[1]);
   const int sz = DIM == 3 ? (int)(r / (uint32_t)g.S[1]) : 0;
-  const int x0 = sx * sw, y0 = sy * sw, z0 = sz * sw;
-  const int hx = min(sw, g.G[0] - x0);
-  const int hy = min(sw, g.G[1] - y0);
-  const int hz = DIM == 3 ? min(sw, g.G[2] - z0) : 1;
+  const int h0[3] = {sx * sw, sy * sw, sz * sw};
+  const int hx = min(sw, g.G[0] - h0[0]);
+  const int hy = min(sw, g.G[1] - h0[1]);
+  const int hz = DIM == 3 ? min(sw, g.G[2] - h0[2]) : 1;
   const int HP = hx * hy * hz;
-  const float2* scr = a.scratch + (size_t)c * PD;
-  const size_t cstride = (size_t)a.n_coils * PD;
-  for (int e = threadIdx.x; e < HP; e += blockDim.x) {
-    const int lx = e % hx;
-    const int lyz = e / hx;
-    const int ly = lyz % hy;
-    const int lz = lyz / hy;
-    const int q0 = x0 + lx, q1 = y0 + ly, q2 = z0 + lz;
-    float accx = 0.0f, accy = 0.0f;
-    const uint32_t bz = DIM == 3 ? a.cover.off[2][q2] : 0, ez = DIM == 3 ? a.cover.off[2][q2 + 1] : 1;
-    const uint32_t by = a.cover.off[1][q1], ey = a.cover.off[1][q1 + 1];
-    const uint32_t bx = a.cover.off[0][q0], ex = a.cover.off[0][q0 + 1];
-    for (uint32_t kz = bz; kz < ez; kz++) {
-      const uint32_t vz = DIM == 3 ? a.cover.ent[2][kz] : 0u;
-      for (uint32_t ky = by; ky < ey; ky++) {
-        const uint32_t vy = a.cover.ent[1][ky];
-        const uint32_t nyz = (vy >> 8) + (uint32_t)g.S[1] * (vz >> 8);
-        const int iyz = (int)(vy & 255u) + P * (int)(vz & 255u);
-        for (uint32_t kx = bx; kx < ex; kx++) {
-          const uint32_t vx = a.cover.ent[0][kx];
-          const uint32_t n = (vx >> 8) + (uint32_t)g.S[0] * nyz;
-          const int pt = (int)(vx & 255u) + P * iyz;
-          const uint32_t k1 = a.sector_chunks[n + 1];
-          for (uint32_t k = a.sector_chunks[n]; k < k1; k++) {
-            const float2 v = scr[(size_t)k * cstride + pt];
-            accx += v.x;
-            accy += v.y;
-          }
+  const int rx_n = (P + g.G[0] - 1) / g.G[0];
+  const int ry_n = (P + g.G[1] - 1) / g.G[1];
+  const int rz_n = DIM == 3 ? (P + g.G[2] - 1) / g.G[2] : 1;
+  const uint32_t cz0 = DIM == 3 ? a.cover.off[2][sz] : 0, cz1 = DIM == 3 ? a.cover.off[2][sz + 1] : 1;
+  const uint32_t cy0 = a.cover.off[1][sy], cy1 = a.cover.off[1][sy + 1];
+  const uint32_t cx0 = a.cover.off[0][sx], cx1 = a.cover.off[0][sx + 1];
+  const int ncx = (int)(cx1 - cx0), ncy = (int)(cy1 - cy0), ncz = (int)(cz1 - cz0);
+  const int nreps = rx_n * ry_n * rz_n;
+  const int J = ncx * ncy * ncz * nreps;  // candidates, fixed (z, y, x, rep) order
+  const int tid = threadIdx.x;
+
+  int4 mine = make_int4(0, 0, 0, 0);
+  uint32_t my_rep = 0;
+  bool my_live = false;
+  if (tid < J) {
+    int j = tid;
+    const int rr = j % nreps;
+    j /= nreps;
+    const int jx = j % ncx;
+    j /= ncx;
+    const int jy = j % ncy;
+    const int jz = j / ncy;
+    const int nx = (int)a.cover.ent[0][cx0 + jx];
+    const int ny = (int)a.cover.ent[1][cy0 + jy];
+    const int nz = DIM == 3 ? (int)a.cover.ent[2][cz0 + jz] : 0;
+    const int n = nx + g.S[0] * (ny + g.S[1] * nz);
+    const int k0 = (int)a.sector_chunks[n];
+    my_live = (int)a.sector_chunks[n + 1] > k0;
+    mine = make_int4(nx * sw + sw / 2 - g.off, ny * sw + sw / 2 - g.off, nz * sw + sw / 2 - g.off, k0);
+    my_rep = (uint32_t)(rr % rx_n) | ((uint32_t)((rr / rx_n) % ry_n) << 8) |
+             ((uint32_t)(rr / (rx_n * ry_n)) << 16);
+    slot[tid] = my_live ? 1 : 0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int j = 0; j < J; j++) {
+      const int c = slot[j];
+      slot[j] = acc;
+      acc += c;
+    }
+    slot[J] = acc;
+  }
+  __syncthreads();
+  if (tid < J && my_live) {
+    items[slot[tid]] = mine;
+    reps_of[slot[tid]] = my_rep;
+  }
+  __syncthreads();
+  const int n_items = slot[J];
+
+  const int nt = (int)blockDim.x;
+  const size_t kstride = (size_t)a.n_coils * PD;
+  for (int base = 0; base < HP; base += nt * kFoldPts) {
+    int q[kFoldPts][3];
+    bool live[kFoldPts];
+    float ax[kFoldPts][CB], ay[kFoldPts][CB];
+#pragma unroll
+    for (int p = 0; p < kFoldPts; p++) {
+      const int e = base + p * nt + tid;
+      live[p] = e < HP;
+      const int lx = e % hx;
+      const int lyz = e / hx;
+      q[p][0] = h0[0] + lx;
+      q[p][1] = h0[1] + lyz % hy;
+      q[p][2] = DIM == 3 ? h0[2] + lyz / hy : 0;
+#pragma unroll
+      for (int c = 0; c < CB; c++) {
+        ax[p][c] = 0.0f;
+        ay[p][c] = 0.0f;
+      }
+    }
+    // Loads are unconditional (points outside a candidate's padded region
+    // read its first word and add zero), so the loads of several items issue
+    // back to back instead of one branch-guarded round trip each.
+#pragma unroll 2
+    for (int it = 0; it < n_items; it++) {
+      const int4 v = items[it];
+      const uint32_t rp = reps_of[it];
+      const float2* src = a.scratch + (size_t)v.w * kstride + (size_t)coil0 * PD;
+#pragma unroll
+      for (int p = 0; p < kFoldPts; p++) {
+        const int ix = wrap(q[p][0] - v.x, g.G[0]) + (int)(rp & 255u) * g.G[0];
+        const int iy = wrap(q[p][1] - v.y, g.G[1]) + (int)((rp >> 8) & 255u) * g.G[1];
+        const int iz = DIM == 3 ? wrap(q[p][2] - v.z, g.G[2]) + (int)(rp >> 16) * g.G[2] : 0;
+        const bool in = live[p] && ix < P && iy < P && iz < P;
+        const int pt = in ? ix + P * (iy + P * iz) : 0;
+        const float m = in ? 1.0f : 0.0f;
+#pragma unroll
+        for (int c = 0; c < CB; c++) {
+          const float2 val = src[(size_t)min(c, a.n_coils - 1 - coil0) * PD + pt];
+          ax[p][c] += m * val.x;
+          ay[p][c] += m * val.y;
         }
       }
     }
-    int o0 = q0, o1 = q1, o2 = q2;
-    if (a.fold_shift) {
-      o0 = wrap(o0 - g.shift_inv[0], g.G[0]);
-      o1 = wrap(o1 - g.shift_inv[1], g.G[1]);
-      if (DIM == 3) o2 = wrap(o2 - g.shift_inv[2], g.G[2]);
+#pragma unroll
+    for (int p = 0; p < kFoldPts; p++) {
+      if (!live[p]) continue;
+      int o0 = q[p][0], o1 = q[p][1], o2 = q[p][2];
+      if (a.fold_shift) {
+        o0 = wrap(o0 - g.shift_inv[0], g.G[0]);
+        o1 = wrap(o1 - g.shift_inv[1], g.G[1]);
+        if (DIM == 3) o2 = wrap(o2 - g.shift_inv[2], g.G[2]);
+      }
+      const size_t gi = (size_t)o0 + (size_t)g.G[0] * ((size_t)o1 + (size_t)g.G[1] * o2);
+#pragma unroll
+      for (int c = 0; c < CB; c++)
+        if (coil0 + c < a.n_coils) a.grid[(size_t)(coil0 + c) * g.Ng + gi] = make_float2(ax[p][c], ay[p][c]);
     }
-    a.grid[(size_t)c * g.Ng + (size_t)o0 + (size_t)g.G[0] * ((size_t)o1 + (size_t)g.G[1] * o2)] =
-        make_float2(accx, accy);
   }
 }
 
-void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6]) {
+template <int DIM, int CB>
+__global__ void __launch_bounds__(kFoldThreads) fold_kernel(Geom g, SpreadArgs a) {
+  fold_body<DIM, CB>(g, a, blockIdx.x, blockIdx.y);
+}
+
+void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3]) {
   std::vector<uint32_t>& w = *words;
   w.clear();
   std::vector<std::vector<uint32_t>> lists[3];
   for (int a = 0; a < 3; a++) {
-    const int G = g.G[a];
-    lists[a].assign((size_t)G, {});
+    const int G = g.G[a], S = g.S[a];
+    lists[a].assign((size_t)S, {});
     if (a >= g.dim) {
-      lists[a][0].push_back(0u);  // 2-D: the single z plane, padded index 0
+      lists[a][0].push_back(0u);  // 2-D: the single z plane
       continue;
     }
-    for (int n = 0; n < g.S[a]; n++) {
-      const int first = n * g.sw + g.sw / 2 - g.off;  // sector center - off
-      for (int i = 0; i < g.P; i++) {
-        const int q = ((first + i) % G + G) % G;
-        lists[a][(size_t)q].push_back(((uint32_t)n << 8) | (uint32_t)i);
+    for (int s = 0; s < S; s++) {
+      const int lo = s * g.sw, hi = std::min(s * g.sw + g.sw, G);
+      for (int n = 0; n < S; n++) {
+        const int first = n * g.sw + g.sw / 2 - g.off;  // sector center - off
+        bool meets = false;
+        for (int i = 0; i < g.P && !meets; i++) {
+          const int q = ((first + i) % G + G) % G;
+          meets = q >= lo && q < hi;
+        }
+        if (meets) lists[a][(size_t)s].push_back((uint32_t)n);
       }
     }
   }
   for (int a = 0; a < 3; a++) {
     base[a] = w.size();
+    max_len[a] = 0;
     uint32_t acc = 0;
     for (const auto& l : lists[a]) {
       w.push_back(acc);
       acc += (uint32_t)l.size();
+      max_len[a] = std::max(max_len[a], (int)l.size());
     }
     w.push_back(acc);
   }
@@ -444,7 +646,8 @@ void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6]) {
 // Interp
 // ---------------------------------------------------------------------------
 template <int DIM, int NX, int CB>
-__global__ void __launch_bounds__(kThreads) interp_kernel(Geom g, InterpArgs a, int R) {
+__device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t ci, int coil_block,
+                                   int R) {
   constexpr int NT = kThreads;
   constexpr int NW = NT / 64;
   constexpr int SPW = 64 / CB;  // sample slots per wave; one lane per (slot, coil)
@@ -462,11 +665,9 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(Geom g, InterpArgs a, 
   const int KPS = P + NX;
   const int PD = DIM == 3 ? P * P * P : P * P;
 
-  const uint32_t ci = work_chunk(a.work, blockIdx.x);
-  if (ci == ~0u) return;
   const Chunk ch = a.work.chunks[ci];
   const int n = (int)(ch.end - ch.begin);
-  const int coil0 = blockIdx.y * CB;
+  const int coil0 = coil_block * CB;
   const int tid = threadIdx.x;
   int c3[3];
   load_center<DIM>(a.centers, ch.sector, c3);
@@ -548,6 +749,13 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(Geom g, InterpArgs a, 
       a.out[(size_t)cg * a.coil_stride + dst] = make_float2(rx, ry);
     }
   }
+}
+
+template <int DIM, int NX, int CB>
+__global__ void __launch_bounds__(kThreads) interp_kernel(Geom g, InterpArgs a, int R) {
+  const uint32_t ci = work_chunk(a.work, blockIdx.x);
+  if (ci == ~0u) return;
+  interp_body<DIM, NX, CB>(g, a, ci, blockIdx.y, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -636,8 +844,11 @@ static hipError_t interp_launch(const Geom& g, const InterpArgs& a, int R, hipSt
 
 template <int DIM, int NX>
 static hipError_t spread_nx(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s) {
+  // Prefer a footprint that lets two workgroups share a CU.
   int R = 0;
-  switch (pick_cb<SpreadLayout>(g, a.n_coils, 8, chunk, lds_limit(), &R)) {
+  int cb = pick_cb<SpreadLayout>(g, a.n_coils, 8, chunk, lds_limit() / 2 - 1024, &R);
+  if (cb == 0) cb = pick_cb<SpreadLayout>(g, a.n_coils, 8, chunk, lds_limit() - 1024, &R);
+  switch (cb) {
     case 8: return spread_launch<DIM, NX, 8>(g, a, R, s);
     case 4: return spread_launch<DIM, NX, 4>(g, a, R, s);
     case 2: return spread_launch<DIM, NX, 2>(g, a, R, s);
@@ -661,6 +872,19 @@ static hipError_t interp_nx(const Geom& g, const InterpArgs& a, int chunk, hipSt
   }
 }
 
+// Fold workgroup size: enough threads for kFoldPts home points each and one
+// per candidate sector while the item list is built; 0 if too many candidates.
+int fold_threads(const Geom& g, const Cover& cover) {
+  const int hp = g.dim == 3 ? g.sw * g.sw * g.sw : g.sw * g.sw;
+  const int per = (hp + kFoldPts - 1) / kFoldPts;
+  int threads = per >= kFoldThreads ? kFoldThreads : ((per + 63) / 64) * 64;
+  int cands = 1;
+  for (int d = 0; d < g.dim; d++) cands *= cover.max_len[d] * ((g.P + g.G[d] - 1) / g.G[d]);
+  if (cands > kFoldCands) return 0;
+  if (cands > threads) threads = ((cands + 63) / 64) * 64;
+  return threads;
+}
+
 hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s) {
   if (a.n_coils == 0) return hipSuccess;
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
@@ -672,14 +896,36 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStrea
       e = nx_of(g) == 4 ? spread_nx<2, 4>(g, a, chunk, s) : spread_nx<2, 8>(g, a, chunk, s);
     if (e != hipSuccess) return e;
   }
+  const int PDn = g.dim == 3 ? g.P * g.P * g.P : g.P * g.P;
+  if (a.n_reduce > 0) {
+    const size_t kstride = (size_t)a.n_coils * PDn;
+    const dim3 rgrid((unsigned)((kstride + 255) / 256), a.n_reduce);
+    hipLaunchKernelGGL(reduce_chunks_kernel, rgrid, dim3(256), 0, s, a.reduce_tasks, a.scratch,
+                       kstride);
+  }
   // the fold writes every grid point, zeros where no sample contributes
   const uint32_t S = (uint32_t)g.S[0] * g.S[1] * g.S[2];
   const int hp = g.dim == 3 ? g.sw * g.sw * g.sw : g.sw * g.sw;
-  const int threads = hp >= 256 ? 256 : ((hp + 63) / 64) * 64;
-  if (g.dim == 3)
-    hipLaunchKernelGGL(fold_kernel<3>, dim3(S, a.n_coils), dim3(threads), 0, s, g, a);
-  else
-    hipLaunchKernelGGL(fold_kernel<2>, dim3(S, a.n_coils), dim3(threads), 0, s, g, a);
+  const int threads = fold_threads(g, a.cover);
+  if (threads == 0) return hipErrorInvalidValue;
+  (void)hp;
+  const int cb = a.n_coils >= 8 ? 8 : a.n_coils >= 4 ? 4 : a.n_coils >= 2 ? 2 : 1;
+  const dim3 grid(S, (a.n_coils + cb - 1) / cb);
+  if (g.dim == 3) {
+    switch (cb) {
+      case 8: hipLaunchKernelGGL((fold_kernel<3, 8>), grid, dim3(threads), 0, s, g, a); break;
+      case 4: hipLaunchKernelGGL((fold_kernel<3, 4>), grid, dim3(threads), 0, s, g, a); break;
+      case 2: hipLaunchKernelGGL((fold_kernel<3, 2>), grid, dim3(threads), 0, s, g, a); break;
+      default: hipLaunchKernelGGL((fold_kernel<3, 1>), grid, dim3(threads), 0, s, g, a); break;
+    }
+  } else {
+    switch (cb) {
+      case 8: hipLaunchKernelGGL((fold_kernel<2, 8>), grid, dim3(threads), 0, s, g, a); break;
+      case 4: hipLaunchKernelGGL((fold_kernel<2, 4>), grid, dim3(threads), 0, s, g, a); break;
+      case 2: hipLaunchKernelGGL((fold_kernel<2, 2>), grid, dim3(threads), 0, s, g, a); break;
+      default: hipLaunchKernelGGL((fold_kernel<2, 1>), grid, dim3(threads), 0, s, g, a); break;
+    }
+  }
   return hipGetLastError();
 }
 
@@ -688,6 +934,106 @@ hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStrea
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
   if (g.dim == 3) return nx_of(g) == 4 ? interp_nx<3, 4>(g, a, chunk, s) : interp_nx<3, 8>(g, a, chunk, s);
   return nx_of(g) == 4 ? interp_nx<2, 4>(g, a, chunk, s) : interp_nx<2, 8>(g, a, chunk, s);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: one launch per stage for a batch of operators
+// ---------------------------------------------------------------------------
+template <int DIM, int NX>
+__global__ void __launch_bounds__(kThreads) spread_batch_kernel(const BatchOp* __restrict__ ops,
+                                                                BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  spread_body<DIM, NX, kBatchCB>(op.g, op.s, it.y, (int)it.z, op.R_spread);
+}
+
+template <int DIM, int NX>
+__global__ void __launch_bounds__(kThreads) interp_batch_kernel(const BatchOp* __restrict__ ops,
+                                                                BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  interp_body<DIM, NX, kBatchCB>(op.g, op.i, it.y, (int)it.z, op.R_interp);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kFoldThreads) fold_batch_kernel(const BatchOp* __restrict__ ops,
+                                                                  BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  fold_body<DIM, kBatchCB>(op.g, op.s, it.y, (int)it.z);
+}
+
+// task (operator, first chunk, chunk count, 0) per blockIdx.y
+__global__ void __launch_bounds__(256) reduce_batch_kernel(const BatchOp* __restrict__ ops,
+                                                           const uint4* __restrict__ tasks) {
+  const uint4 t = tasks[blockIdx.y];
+  const BatchOp& op = ops[t.x];
+  const size_t PD = op.g.dim == 3 ? (size_t)op.g.P * op.g.P * op.g.P : (size_t)op.g.P * op.g.P;
+  reduce_body(op.s.scratch, (size_t)op.s.n_coils * PD, t.y, t.z,
+              (size_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+size_t lds_budget() { return lds_limit(); }
+
+size_t spread_lds_bytes(const Geom& g, int cb, int R) {
+  return SpreadLayout(g, nx_of(g), cb, R).bytes;
+}
+
+size_t interp_lds_bytes(const Geom& g, int cb, int R) {
+  return InterpLayout(g, nx_of(g), cb, R).bytes;
+}
+
+int spread_round(const Geom& g, int cb, int chunk, size_t limit) {
+  return round_size<SpreadLayout>(g, cb, chunk, limit);
+}
+
+int interp_round(const Geom& g, int cb, int chunk, size_t limit) {
+  return round_size<InterpLayout>(g, cb, chunk, limit);
+}
+
+template <typename K>
+static hipError_t launch_items(K kernel, const BatchOp* ops, const BatchList& l, int threads,
+                               size_t lds, hipStream_t s) {
+  if (l.n == 0) return hipSuccess;
+  hipError_t e = allow_lds(kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kernel, dim3(batch_blocks(l)), dim3(threads), lds, s, ops, l);
+  return hipGetLastError();
+}
+
+hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
+                               size_t lds, hipStream_t s) {
+  if (dim == 3)
+    return nx == 4 ? launch_items(spread_batch_kernel<3, 4>, ops, l, kThreads, lds, s)
+                   : launch_items(spread_batch_kernel<3, 8>, ops, l, kThreads, lds, s);
+  return nx == 4 ? launch_items(spread_batch_kernel<2, 4>, ops, l, kThreads, lds, s)
+                 : launch_items(spread_batch_kernel<2, 8>, ops, l, kThreads, lds, s);
+}
+
+hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
+                               size_t lds, hipStream_t s) {
+  if (dim == 3)
+    return nx == 4 ? launch_items(interp_batch_kernel<3, 4>, ops, l, kThreads, lds, s)
+                   : launch_items(interp_batch_kernel<3, 8>, ops, l, kThreads, lds, s);
+  return nx == 4 ? launch_items(interp_batch_kernel<2, 4>, ops, l, kThreads, lds, s)
+                 : launch_items(interp_batch_kernel<2, 8>, ops, l, kThreads, lds, s);
+}
+
+hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, int threads,
+                             hipStream_t s) {
+  return dim == 3 ? launch_items(fold_batch_kernel<3>, ops, l, threads, 0, s)
+                  : launch_items(fold_batch_kernel<2>, ops, l, threads, 0, s);
+}
+
+hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
+                               hipStream_t s) {
+  if (l.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_batch_kernel, dim3((max_kstride + 255) / 256, l.n), dim3(256), 0, s,
+                     ops, l.items);
+  return hipGetLastError();
 }
 
 }  // namespace gnufft

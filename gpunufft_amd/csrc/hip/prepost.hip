@@ -47,13 +47,10 @@ __device__ inline void pixel_coords(const Geom& g, long long p, int* x, int* y, 
 //   otherwise:    v_c = grid[c][m(p)] * scale * deapo[p]
 //                 sens:   out[p] (+)= sum_c conj(S_c[p]) v_c   (accumulate: add to out)
 //                 no sens: out[c][p] = v_c
-__global__ void __launch_bounds__(256) adjoint_post_kernel(Geom g, const float2* __restrict__ grid,
-                                                           int n_coils,
-                                                           const float* __restrict__ deapo,
-                                                           const float2* __restrict__ sens,
-                                                           float2* __restrict__ out,
-                                                           int accumulate, int stop) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void adjoint_post_body(const Geom& g, const float2* __restrict__ grid,
+                                         int n_coils, const float* __restrict__ deapo,
+                                         const float2* __restrict__ sens, float2* __restrict__ out,
+                                         int accumulate, int stop, long long p) {
   if (p >= g.Ni) return;
   int x, y, z;
   pixel_coords(g, p, &x, &y, &z);
@@ -89,6 +86,16 @@ __global__ void __launch_bounds__(256) adjoint_post_kernel(Geom g, const float2*
   }
 }
 
+__global__ void __launch_bounds__(256) adjoint_post_kernel(Geom g, const float2* __restrict__ grid,
+                                                           int n_coils,
+                                                           const float* __restrict__ deapo,
+                                                           const float2* __restrict__ sens,
+                                                           float2* __restrict__ out,
+                                                           int accumulate, int stop) {
+  adjoint_post_body(g, grid, n_coils, deapo, sens, out, accumulate, stop,
+                    (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 hipError_t launch_adjoint_post(const Geom& g, const float2* grid, int n_coils,
                                const float* deapo, const float2* sens, float2* out,
                                bool accumulate, int stop, hipStream_t s) {
@@ -100,12 +107,10 @@ hipError_t launch_adjoint_post(const Geom& g, const float2* grid, int n_coils,
 
 // Forward head: [x S_c] x deapo, zero-pad, INVERSE fftshift: writes every
 // grid point of every coil (no separate memset).
-__global__ void __launch_bounds__(256) forward_pre_kernel(Geom g, const float2* __restrict__ img,
-                                                          int img_per_coil,
-                                                          const float2* __restrict__ sens,
-                                                          const float* __restrict__ deapo,
-                                                          float2* __restrict__ grid, int n_coils) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void forward_pre_body(const Geom& g, const float2* __restrict__ img,
+                                        int img_per_coil, const float2* __restrict__ sens,
+                                        const float* __restrict__ deapo, float2* __restrict__ grid,
+                                        int n_coils, long long t) {
   if (t >= g.Ng) return;
   const int tx = (int)(t % g.G[0]);
   const long long r = t / g.G[0];
@@ -132,6 +137,15 @@ __global__ void __launch_bounds__(256) forward_pre_kernel(Geom g, const float2* 
     }
     grid[(size_t)c * g.Ng + t] = make_float2(v.x * dp, v.y * dp);
   }
+}
+
+__global__ void __launch_bounds__(256) forward_pre_kernel(Geom g, const float2* __restrict__ img,
+                                                          int img_per_coil,
+                                                          const float2* __restrict__ sens,
+                                                          const float* __restrict__ deapo,
+                                                          float2* __restrict__ grid, int n_coils) {
+  forward_pre_body(g, img, img_per_coil, sens, deapo, grid, n_coils,
+                   (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 hipError_t launch_forward_pre(const Geom& g, const float2* img, bool img_per_coil,
@@ -172,6 +186,38 @@ hipError_t launch_image_prep(const Geom& g, const float2* img, bool img_per_coil
   unsigned blocks = (unsigned)((g.Ni + 255) / 256);
   hipLaunchKernelGGL(image_prep_kernel, dim3(blocks), dim3(256), 0, s, g, img,
                      img_per_coil ? 1 : 0, sens, deapo, out, n_coils);
+  return hipGetLastError();
+}
+
+// Batched: item (operator, block of 256 elements, 0, 0) per workgroup; the
+// batch list is not XCD-partitioned (elementwise, no locality to keep).
+__global__ void __launch_bounds__(256) post_batch_kernel(const BatchOp* __restrict__ ops,
+                                                         const uint4* __restrict__ items) {
+  const uint4 it = items[blockIdx.x];
+  const BatchOp& op = ops[it.x];
+  const PostArgs& a = op.post;
+  adjoint_post_body(op.g, a.grid, a.n_coils, a.deapo, a.sens, a.out, 0, a.stop,
+                    (long long)it.y * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) pre_batch_kernel(const BatchOp* __restrict__ ops,
+                                                        const uint4* __restrict__ items) {
+  const uint4 it = items[blockIdx.x];
+  const BatchOp& op = ops[it.x];
+  const PreArgs& a = op.pre;
+  forward_pre_body(op.g, a.img, a.img_per_coil, a.sens, a.deapo, a.grid, a.n_coils,
+                   (long long)it.y * 256 + threadIdx.x);
+}
+
+hipError_t launch_post_batch(const BatchOp* ops, const BatchList& l, hipStream_t s) {
+  if (l.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(post_batch_kernel, dim3(l.n), dim3(256), 0, s, ops, l.items);
+  return hipGetLastError();
+}
+
+hipError_t launch_pre_batch(const BatchOp* ops, const BatchList& l, hipStream_t s) {
+  if (l.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(pre_batch_kernel, dim3(l.n), dim3(256), 0, s, ops, l.items);
   return hipGetLastError();
 }
 

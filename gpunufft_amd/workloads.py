@@ -80,6 +80,31 @@ def gaussian_sens(img_dims, n_coils, seed, sigma=0.3, ring=0.35):
     return s.astype(np.complex64)
 
 
+BATCH_SIZES = (96, 128, 192, 256, 320, 384)
+
+
+def uneven_batch_2d(n_ops=64, seed0=100):
+    """Batched uneven 2-D workload: n_ops independent operators, operator j
+    seeded with seed0 + j. Image N uniform over BATCH_SIZES; variable-density
+    spiral k(t) = 0.5 t^1.5 exp(i(2 pi n_turns t^1.5 + 2 pi l / L)) with
+    L in [8, 48] interleaves of [1024, 4096] samples and n_turns = N / (2 L)
+    (one interleave-turn spacing of L/N in k); coils in [4, 16]; osf 2,
+    kernel width 3, |k| density, Gaussian-blob coil maps.
+    Returns a list of dicts (traj, img_dims, coils, dens, sens, osf, kw, seed)."""
+    ops = []
+    for j in range(n_ops):
+        rng = np.random.default_rng(seed0 + j)
+        N = int(rng.choice(BATCH_SIZES))
+        L = int(rng.integers(8, 49))
+        ns = int(rng.integers(1024, 4097))
+        C = int(rng.integers(4, 17))
+        traj = spiral_2d(L, ns, n_turns=max(1.0, N / (2.0 * L)))
+        ops.append(dict(traj=traj, img_dims=(N, N), coils=C, dens=ramp_density(traj),
+                        sens=gaussian_sens((N, N), C, seed=seed0 + j), osf=2.0, kw=3,
+                        seed=seed0 + j))
+    return ops
+
+
 def config(name):
     """Workload of a named configuration: dict(traj, img_dims, osf, kw, sw, coils, dens, seed)."""
     if name == "radial2d_64":

@@ -186,6 +186,30 @@ int gnufft_spread_sorted(gnufft_plan plan, const void* data_sorted, void* grid,
 int gnufft_interp_sorted(gnufft_plan plan, const void* grid, void* data_sorted,
                          uint32_t n_coils, void* stream);
 
+/* ---- batches of independent operators --------------------------------------
+ * The reference runs operators one at a time, each stage a launch of its
+ * own, with coils in chunks (CUDA/src/gpuNUFFT_operator.cpp:327-368, the
+ * balanced sector order of CUDA/src/gpuNUFFT_operator_factory.cpp:74-126).
+ * A batch runs the full adjoint or forward of many plans (any image sizes
+ * and trajectory lengths; one dimension and kernel-width class) with one
+ * launch per stage: spread, chunk reduction, fold, post, pre and interp
+ * kernels take (plan, work item, coil block) tiles from one list balanced
+ * over the XCDs; FFTs run once per distinct grid size, batched over all
+ * coils of those plans.
+ * n_coils[i]: coils of plan i (its sensitivity coil count when it has
+ * sensitivities). The batch keeps the plans' device tables by reference:
+ * plans must outlive it and keep their trajectories. */
+typedef struct gnufft_batch_s* gnufft_batch;
+int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t n_plans,
+                        const uint32_t* n_coils, void* stream);
+int gnufft_batch_destroy(gnufft_batch batch);
+/* kdata[i] / img[i] and out[i]: device buffers of plan i laid out as for
+ * gnufft_adjoint / gnufft_forward with stop DEAPODIZATION. */
+int gnufft_batch_adjoint(gnufft_batch batch, const void* const* kdata, void* const* out,
+                         void* stream);
+int gnufft_batch_forward(gnufft_batch batch, const void* const* img, void* const* out,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

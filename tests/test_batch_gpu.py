@@ -1,0 +1,103 @@
+"""Batches of independent plans (gnufft_batch_*): every plan's batched
+adjoint and forward equal its own single-plan adjoint and forward and the
+oracle (rel-L2 <= 1e-5), for uneven image sizes, trajectory lengths, coil
+counts, with and without sensitivities, in 2-D and 3-D."""
+
+import numpy as np
+import pytest
+
+from gpunufft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a ROCm device"
+    from gpunufft_amd import capi
+    capi.load()
+    return torch, capi
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _ops_2d():
+    ops = W.uneven_batch_2d(n_ops=7, seed0=500)
+    # shrink to test size: fewer samples, smaller images, keep the variety
+    out = []
+    for j, o in enumerate(ops):
+        N = [48, 64, 40, 96, 64, 32, 80][j]
+        traj = W.spiral_2d(4 + j, 300 + 97 * j, n_turns=N / 10.0)
+        if j == 3:  # heavy centre: sectors split over several chunks
+            rng = np.random.default_rng(9)
+            traj = np.concatenate([traj, rng.uniform(-0.02, 0.02, (2, 5000)).astype(np.float32)], 1)
+        C = [1, 3, 4, 6, 2, 5, 8][j]
+        sens = W.gaussian_sens((N, N), C, seed=j) if j % 2 == 0 else None
+        out.append(dict(traj=traj, img=(N, N), C=C, sens=sens, dens=W.ramp_density(traj), sw=8))
+    return out
+
+
+def _ops_3d():
+    out = []
+    for j, (N, C) in enumerate([(16, 2), (24, 3), (20, 1)]):
+        traj = W.koosh_3d(96 + 32 * j, 40)
+        sens = W.gaussian_sens((N, N, N), C, seed=j) if j != 1 else None
+        out.append(dict(traj=traj, img=(N, N, N), C=C, sens=sens, dens=W.ramp_density(traj, 2), sw=8))
+    return out
+
+
+@pytest.mark.parametrize("make", [_ops_2d, _ops_3d], ids=["2d", "3d"])
+def test_batch_equals_single_plans(oracle, env, make):
+    torch, capi = env
+    specs = make()
+    rng = np.random.default_rng(1)
+    plans, kd, img, shapes = [], [], [], []
+    for sp in specs:
+        p = capi.Plan(sp["img"], 3, sp["sw"], 2.0 if len(sp["img"]) == 2 else 1.5)
+        p.set_trajectory(_dev(torch, sp["traj"]), _dev(torch, sp["dens"]))
+        if sp["sens"] is not None:
+            p.set_sens(_dev(torch, sp["sens"]), sp["C"])
+        plans.append(p)
+        M = sp["traj"].shape[1]
+        ishape = tuple(sp["img"][::-1])
+        kd.append(_dev(torch, W.complex_normal(rng, (sp["C"], M))))
+        img.append(_dev(torch, W.complex_normal(rng, ((1,) if sp["sens"] is not None else (sp["C"],)) + ishape)))
+        shapes.append(((1,) if sp["sens"] is not None else (sp["C"],)) + ishape)
+    print("batch of", len(plans), [s["img"] for s in specs], flush=True)
+    b = capi.Batch(plans, [s["C"] for s in specs])
+    a_out = [torch.empty(sh, dtype=torch.complex64, device="cuda") for sh in shapes]
+    f_out = [torch.empty(k.shape, dtype=torch.complex64, device="cuda") for k in kd]
+    b.adjoint(kd, a_out)
+    b.forward(img, f_out)
+    # twice: descriptors are re-uploaded per call
+    b.adjoint(kd, a_out)
+    torch.cuda.synchronize()
+    for j, (p, sp) in enumerate(zip(plans, specs)):
+        ref_a = torch.empty(shapes[j], dtype=torch.complex64, device="cuda")
+        p.adjoint(kd[j], ref_a, sp["C"])
+        ref_f = torch.empty(kd[j].shape, dtype=torch.complex64, device="cuda")
+        p.forward(img[j], ref_f, sp["C"])
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(a_out[j].cpu().numpy(), ref_a.cpu().numpy()) < TOL, ("adjoint", j)
+        assert oracle.rel_l2(f_out[j].cpu().numpy(), ref_f.cpu().numpy()) < TOL, ("forward", j)
+    # one plan against the oracle as well
+    j = len(specs) - 1
+    sp = specs[j]
+    op = oracle.Operator(sp["traj"], sp["img"], 3, sp["sw"], 2.0 if len(sp["img"]) == 2 else 1.5,
+                         dens=sp["dens"], sens=sp["sens"], order="stable")
+    assert oracle.rel_l2(a_out[j].cpu().numpy(), op.adjoint(kd[j].cpu().numpy())) < TOL
+    assert oracle.rel_l2(f_out[j].cpu().numpy(), op.forward(img[j].cpu().numpy())) < TOL
+
+
+def test_batch_rejects_mixed_dimensions(env):
+    torch, capi = env
+    p2 = capi.Plan((16, 16), 3, 8, 2.0)
+    p2.set_trajectory(_dev(torch, W.radial_2d(4, 16)))
+    p3 = capi.Plan((8, 8, 8), 3, 8, 2.0)
+    p3.set_trajectory(_dev(torch, W.koosh_3d(8, 8)))
+    with pytest.raises(capi.GnufftError):
+        capi.Batch([p2, p3], [1, 1])
