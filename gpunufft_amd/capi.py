@@ -28,7 +28,8 @@ EXPORTS = [
     "gnufft_plan_get_assigned_sectors", "gnufft_adjoint", "gnufft_forward",
     "gnufft_spread_sorted", "gnufft_interp_sorted", "gnufft_plan_set_timing",
     "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
-    "gnufft_batch_adjoint", "gnufft_batch_forward",
+    "gnufft_batch_adjoint", "gnufft_batch_forward", "gnufft_cg_create", "gnufft_cg_destroy",
+    "gnufft_cg_solve",
 ]
 
 T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
@@ -118,6 +119,9 @@ def load(path=LIB_PATH):
     L.gnufft_batch_destroy.argtypes = [vp]
     L.gnufft_batch_adjoint.argtypes = [vp, vpp, vpp, vp]
     L.gnufft_batch_forward.argtypes = [vp, vpp, vpp, vp]
+    L.gnufft_cg_create.argtypes = [ctypes.POINTER(vp), vp, u32, f32]
+    L.gnufft_cg_destroy.argtypes = [vp]
+    L.gnufft_cg_solve.argtypes = [vp, vp, vp, u32, i32, vp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or i32
     L.gnufft_last_error.restype = ctypes.c_char_p
@@ -302,3 +306,31 @@ class Batch:
         check(load().gnufft_batch_forward(self._h, self._ptrs(img), self._ptrs(out),
                                           _stream_ptr(stream)))
         return out
+
+
+class CGSense:
+    """Device-resident CG-SENSE solver on a plan with sensitivities
+    (gnufft_cg_*): x = argmin of |A x - y|^2 + alpha |x|^2 by plain CG."""
+
+    def __init__(self, plan, n_coils, alpha=0.0):
+        self._plan = plan
+        self._h = ctypes.c_void_p()
+        check(load().gnufft_cg_create(ctypes.byref(self._h), plan.handle, int(n_coils),
+                                      float(alpha)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().gnufft_cg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, kdata, x_out, iterations=10, use_graph=True, stream=None):
+        """kdata (C, M) and x_out (1, *img) device tensors; asynchronous."""
+        check(load().gnufft_cg_solve(self._h, _ptr_loc(kdata)[0], _ptr_loc(x_out)[0],
+                                     int(iterations), 1 if use_graph else 0, _stream_ptr(stream)))
+        return x_out

@@ -32,6 +32,12 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+void gnufft::set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                      \
   do {                                                                                     \
     hipError_t e_ = (expr);                                                                \
@@ -187,6 +193,10 @@ struct gnufft_plan_s {
   Chunk* chunks_d = nullptr;
   uint32_t n_chunks = 0;
   uint32_t chunk = 1024;
+  // samples per LDS round of the gridding kernels: the mean chunk occupancy
+  // rounded up to 64 (sparse trajectories get small footprints, hence more
+  // workgroups per CU), at most `chunk`
+  uint32_t round_target = 1024;
   WorkList work{};
   uint32_t* sector_chunks_d = nullptr;  // [S+1] first chunk of each sector
   uint2* reduce_d = nullptr;            // (first chunk, count) of multi-chunk sectors
@@ -383,6 +393,8 @@ int ensure_scratch(gnufft_plan p, uint32_t coils) {
 }
 
 uint32_t coil_batch(gnufft_plan p, uint32_t n_coils) {
+  // buffers already hold every coil (steady state, e.g. under graph capture)
+  if (p->grid_coils >= n_coils && p->scratch_coils >= n_coils) return n_coils;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 1;
   // grid + FFT work + spread sub-grids
@@ -499,6 +511,10 @@ int build_chunks(gnufft_plan p, hipStream_t s) {
   for (uint32_t i = 0; i < total; i++) counts[i] = h[i].end - h[i].begin;
   p->work.chunks = p->chunks_d;
   partition_work(counts.data(), total, &p->work);
+  {
+    const uint64_t mean = total ? ((uint64_t)p->M + total - 1) / total : 64;
+    p->round_target = (uint32_t)std::min<uint64_t>(p->chunk, std::max<uint64_t>(64, (mean + 63) / 64 * 64));
+  }
   // sectors split over several chunks: their sub-grids are summed before the fold
   std::vector<uint2> tasks;
   for (uint32_t i = 0; i < total;) {
@@ -925,7 +941,7 @@ int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uin
     SpreadArgs a = spread_args(p, (const float2*)data_sorted + (size_t)c0 * p->M,
                                (float2*)grid + (size_t)c0 * p->g.Ng, nc);
     a.fold_shift = false;
-    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, (int)p->chunk, s));
+    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(p->g, a, (int)p->round_target, s));
   }
   return GNUFFT_OK;
 }
@@ -944,7 +960,7 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   a.coil_stride = p->M;
   a.scale = 1.0f;
   a.n_coils = (int)n_coils;
-  TIMED(p, GNUFFT_T_INTERP, (hipStream_t)stream, launch_interp(p->g, a, (int)p->chunk, (hipStream_t)stream));
+  TIMED(p, GNUFFT_T_INTERP, (hipStream_t)stream, launch_interp(p->g, a, (int)p->round_target, (hipStream_t)stream));
   return GNUFFT_OK;
 }
 
@@ -976,7 +992,7 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
     a.gather = p->idx_d;
     a.dens = p->dens_sorted_d;
     a.fold_shift = !grid_out;
-    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, (int)p->chunk, s));
+    TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, (int)p->round_target, s));
     if (grid_out) continue;
     int rc = run_fft(p, nc, true, grid, s);
     if (rc) return rc;
@@ -1027,7 +1043,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.scatter = p->idx_d;
     a.scale = 1.0f;
     a.n_coils = (int)n_coils;
-    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->chunk, s));
+    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->round_target, s));
     return GNUFFT_OK;
   }
   const uint32_t cc = coil_batch(p, n_coils);
@@ -1053,7 +1069,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.scale = g.fft_scale;
     a.n_coils = (int)nc;
     a.fold_shift = true;
-    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->chunk, s));
+    TIMED(p, GNUFFT_T_INTERP, s, launch_interp(g, a, (int)p->round_target, s));
   }
   return GNUFFT_OK;
 }
@@ -1201,8 +1217,8 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
                        p->n_sens ? p->sens_d : nullptr, nullptr, GNUFFT_STOP_DEAPODIZATION};
     op.pre = PreArgs{nullptr, p->n_sens ? 0 : 1, p->n_sens ? p->sens_d : nullptr, p->deapo_d,
                      b->grid_d + grid_off[k], (int)C};
-    op.R_spread = spread_round(p->g, kBatchCB, (int)p->chunk, half);
-    op.R_interp = interp_round(p->g, kBatchCB, (int)p->chunk, half);
+    op.R_spread = spread_round(p->g, kBatchCB, (int)p->round_target, half);
+    op.R_interp = interp_round(p->g, kBatchCB, (int)p->round_target, half);
     if (op.R_spread <= 0 || op.R_interp <= 0) {
       rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: padded sector does not fit in LDS");
       break;

@@ -142,6 +142,9 @@ __device__ __host__ inline int wrap(int p, int G) {
   return p;
 }
 
+// Message returned by gnufft_last_error() on this thread (capi.hip).
+void set_last_error(const std::string& msg);
+
 // ---------------------------------------------------------------------------
 // Launchers implemented in the .hip translation units (all asynchronous on
 // `stream`; return hipError_t).

@@ -210,6 +210,22 @@ int gnufft_batch_adjoint(gnufft_batch batch, const void* const* kdata, void* con
 int gnufft_batch_forward(gnufft_batch batch, const void* const* img, void* const* out,
                          void* stream);
 
+/* ---- device-resident CG-SENSE ----------------------------------------------
+ * The reference's iterative reconstruction (matlab/demo/utils/cg_sense_2d.m:
+ * right-hand side :46-47, system matrix A^H A + alpha I :51 and :99-102,
+ * CG iterations :63-82, useMulticoil) drives its operator from the host with
+ * a host round trip per product. Here the solver keeps x, r, p, A p and its
+ * scalars on the device; one iteration is captured into a HIP graph (after a
+ * warm-up that allocates the plan's work buffers) and replayed.
+ * The plan must have sensitivities of n_coils coils. solve: kdata [n_coils][M]
+ * (device, caller order) -> x_out [img] (device); asynchronous on `stream`.
+ * One solver per stream at a time (it owns its vectors). */
+typedef struct gnufft_cg_s* gnufft_cg;
+int gnufft_cg_create(gnufft_cg* cg, gnufft_plan plan, uint32_t n_coils, float alpha);
+int gnufft_cg_destroy(gnufft_cg cg);
+int gnufft_cg_solve(gnufft_cg cg, const void* kdata, void* x_out, uint32_t iterations,
+                    int use_graph, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
