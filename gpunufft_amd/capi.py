@@ -16,8 +16,11 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "lib", "libgpunufft_hip.so")
 
 OK = 0
+(ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_FFT, ERR_UNSUPPORTED,
+ ERR_STATE) = 1, 2, 3, 4, 5, 6
 LOOKUP_NEAREST, LOOKUP_LINEAR = 0, 1
 CONVOLUTION, FFT, DEAPODIZATION, DENSITY_ESTIMATION = 0, 1, 2, 3
+FFT_FUSED, FFT_ROCFFT = 0, 1
 HOST, DEVICE = 0, 1
 
 EXPORTS = [
@@ -29,7 +32,7 @@ EXPORTS = [
     "gnufft_spread_sorted", "gnufft_interp_sorted", "gnufft_plan_set_timing",
     "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
     "gnufft_batch_adjoint", "gnufft_batch_forward", "gnufft_cg_create", "gnufft_cg_destroy",
-    "gnufft_cg_solve",
+    "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft",
 ]
 
 T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
@@ -112,6 +115,8 @@ def load(path=LIB_PATH):
     L.gnufft_forward.argtypes = [vp, vp, vp, u32, i32, vp]
     L.gnufft_spread_sorted.argtypes = [vp, vp, vp, u32, vp]
     L.gnufft_interp_sorted.argtypes = [vp, vp, vp, u32, vp]
+    L.gnufft_plan_set_fft.argtypes = [vp, i32]
+    L.gnufft_plan_get_fft.argtypes = [vp, ctypes.POINTER(i32)]
     L.gnufft_plan_set_timing.argtypes = [vp, i32]
     L.gnufft_plan_get_timing.argtypes = [vp, ctypes.POINTER(Timing), i32]
     vpp = ctypes.POINTER(vp)
@@ -252,6 +257,16 @@ class Plan:
         check(load().gnufft_spread_sorted(self._h, _ptr_loc(data_sorted)[0], _ptr_loc(grid)[0],
                                           n_coils, _stream_ptr(stream)))
         return grid
+
+    def set_fft(self, kind):
+        """FFT_FUSED (pruned per-axis passes with the pre / post stages
+        fused) or FFT_ROCFFT (rocFFT plus separate pre / post kernels)."""
+        check(load().gnufft_plan_set_fft(self._h, int(kind)))
+
+    def fft(self):
+        k = ctypes.c_int()
+        check(load().gnufft_plan_get_fft(self._h, ctypes.byref(k)))
+        return k.value
 
     def set_timing(self, enable=True):
         check(load().gnufft_plan_set_timing(self._h, 1 if enable else 0))

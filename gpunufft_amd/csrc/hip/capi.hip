@@ -216,6 +216,11 @@ struct gnufft_plan_s {
   float2* img_tmp_d = nullptr;
   uint32_t img_tmp_coils = 0;
   std::map<uint32_t, FftPlans> ffts;
+  // fused FFT (GNUFFT_FFT_FUSED): twiddle tables [axis][sign][G_a]
+  bool fused_ok = false;
+  bool fused = false;
+  float2* fft_tw_d = nullptr;
+  size_t fft_tw_off[3][2] = {};
   StageTimer timer;
 };
 
@@ -375,8 +380,12 @@ size_t padded_points(const Geom& g) {
   return g.dim == 3 ? (size_t)g.P * g.P * g.P : (size_t)g.P * g.P;
 }
 
+// spread sub-grids [chunk][padded point] per coil; the same memory is the
+// fused FFT's intermediate array (used after the fold has consumed it)
 size_t scratch_per_coil(gnufft_plan p) {
-  return (size_t)p->n_chunks * padded_points(p->g) * sizeof(float2);
+  const size_t sub = (size_t)p->n_chunks * padded_points(p->g);
+  const size_t fft = p->fused_ok ? fused_fft_tmp_elems(p->g) : 0;
+  return std::max(sub, fft) * sizeof(float2);
 }
 
 int ensure_scratch(gnufft_plan p, uint32_t coils) {
@@ -692,6 +701,11 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
     if (e != hipSuccess) rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, "plan_create: deapo allocation");
   }
   if (rc == GNUFFT_OK) rc = compute_deapo(p, s);
+  if (rc == GNUFFT_OK && fused_fft_supported(g)) {
+    const std::vector<float2> tw = fused_fft_twiddles(g, p->fft_tw_off);
+    rc = copy_in((void**)&p->fft_tw_d, tw.data(), tw.size() * sizeof(float2), GNUFFT_HOST, s);
+    p->fused_ok = p->fused = rc == GNUFFT_OK;
+  }
   if (rc == GNUFFT_OK) {
     std::vector<uint32_t> words;
     size_t base[6];
@@ -710,6 +724,26 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
     return rc;
   }
   *plan = p;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_set_fft(gnufft_plan p, int kind) {
+  if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_fft: null plan");
+  if (kind == GNUFFT_FFT_ROCFFT) {
+    p->fused = false;
+    return GNUFFT_OK;
+  }
+  if (kind != GNUFFT_FFT_FUSED) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_fft: unknown kind");
+  if (!p->fused_ok)
+    return fail(GNUFFT_ERR_UNSUPPORTED,
+                "set_fft: fused FFT needs grid lengths 2^a 3^b 5^c up to 2048");
+  p->fused = true;
+  return GNUFFT_OK;
+}
+
+int gnufft_plan_get_fft(gnufft_plan p, int* kind) {
+  if (!p || !kind) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_fft: null argument");
+  *kind = p->fused ? GNUFFT_FFT_FUSED : GNUFFT_FFT_ROCFFT;
   return GNUFFT_OK;
 }
 
@@ -745,6 +779,7 @@ int gnufft_plan_destroy(gnufft_plan p) {
   dfree(p->sens_d);
   dfree(p->grid_d);
   dfree(p->img_tmp_d);
+  dfree(p->fft_tw_d);
   delete p;
   return GNUFFT_OK;
 }
@@ -994,6 +1029,14 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
     a.fold_shift = !grid_out;
     TIMED(p, GNUFFT_T_SPREAD, s, launch_spread(g, a, (int)p->round_target, s));
     if (grid_out) continue;
+    if (p->fused) {
+      float2* dst = use_sens ? (float2*)out : (float2*)out + (size_t)c0 * g.Ni;
+      TIMED(p, GNUFFT_T_FFT, s,
+            fused_fft_adjoint(g, grid, p->scratch_d, (int)nc, p->deapo_d,
+                              use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr, dst, stop,
+                              use_sens && c0 > 0, p->fft_tw_d, p->fft_tw_off, s));
+      continue;
+    }
     int rc = run_fft(p, nc, true, grid, s);
     if (rc) return rc;
     if (use_sens) {
@@ -1049,13 +1092,24 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
   const uint32_t cc = coil_batch(p, n_coils);
   int rc = ensure_grid(p, cc);
   if (rc) return rc;
+  if (p->fused) {
+    rc = ensure_scratch(p, cc);
+    if (rc) return rc;
+  }
   for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
     const uint32_t nc = std::min(cc, n_coils - c0);
     const float2* src = use_sens ? (const float2*)img : (const float2*)img + (size_t)c0 * g.Ni;
-    TIMED(p, GNUFFT_T_PREPOST, s, launch_forward_pre(g, src, !use_sens, use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr,
-                               p->deapo_d, p->grid_d, (int)nc, s));
-    rc = run_fft(p, nc, false, p->grid_d, s);
-    if (rc) return rc;
+    const float2* sens = use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr;
+    if (p->fused) {
+      TIMED(p, GNUFFT_T_FFT, s,
+            fused_fft_forward(g, src, !use_sens, sens, p->deapo_d, p->grid_d, p->scratch_d,
+                              (int)nc, p->fft_tw_d, p->fft_tw_off, s));
+    } else {
+      TIMED(p, GNUFFT_T_PREPOST, s, launch_forward_pre(g, src, !use_sens, sens, p->deapo_d,
+                                                       p->grid_d, (int)nc, s));
+      rc = run_fft(p, nc, false, p->grid_d, s);
+      if (rc) return rc;
+    }
     InterpArgs a{};
     a.grid = p->grid_d;
     a.crds = p->traj_sorted_d;

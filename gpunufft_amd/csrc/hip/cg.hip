@@ -162,6 +162,11 @@ struct gnufft_cg_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool warm = false;
+  // the solver's own stream (default priority, non-blocking: the legacy
+  // null stream cannot be captured), ordered after / before the caller's
+  // stream with events
+  hipStream_t own = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 namespace {
@@ -187,6 +192,9 @@ void cg_free(gnufft_cg c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->buf) (void)hipFree(c->buf);
   if (c->s) (void)hipFree(c->s);
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
+  if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
 
@@ -215,6 +223,12 @@ int gnufft_cg_create(gnufft_cg* cg, gnufft_plan plan, uint32_t n_coils, float al
     cg_free(c);
     return cg_fail(GNUFFT_ERR_OUT_OF_MEMORY, "cg_create: device allocation");
   }
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming) != hipSuccess) {
+    cg_free(c);
+    return cg_fail(GNUFFT_ERR_DEVICE, "cg_create: stream / event creation");
+  }
   c->x = c->buf;
   c->r = c->x + c->n;
   c->p = c->r + c->n;
@@ -236,7 +250,12 @@ int gnufft_cg_destroy(gnufft_cg cg) {
 int gnufft_cg_solve(gnufft_cg c, const void* kdata, void* x_out, uint32_t iterations, int use_graph,
                     void* stream) {
   if (!c || !kdata || !x_out) return cg_fail(GNUFFT_ERR_INVALID_ARGUMENT, "cg_solve: null argument");
-  hipStream_t s = (hipStream_t)stream;
+  // the work runs on the solver's stream, after the caller's stream's
+  // earlier work and before its later work
+  const hipStream_t caller = (hipStream_t)stream;
+  const hipStream_t s = c->own;
+  CG_HIP(hipEventRecord(c->ev_in, caller));
+  CG_HIP(hipStreamWaitEvent(s, c->ev_in, 0));
   // right-hand side A^H y, then x = 0, r = p = A^H y, rr = |r|^2
   CG_ABI(gnufft_adjoint(c->plan, kdata, c->y, c->coils, GNUFFT_STOP_DEAPODIZATION, s));
   hipLaunchKernelGGL(cg_init_kernel, dim3(c->blocks), dim3(kCgThreads), 0, s, c->x, c->r, c->p,
@@ -269,6 +288,8 @@ int gnufft_cg_solve(gnufft_cg c, const void* kdata, void* x_out, uint32_t iterat
     }
   }
   CG_HIP(hipMemcpyAsync(x_out, c->x, c->n * sizeof(float2), hipMemcpyDeviceToDevice, s));
+  CG_HIP(hipEventRecord(c->ev_out, s));
+  CG_HIP(hipStreamWaitEvent(caller, c->ev_out, 0));
   return GNUFFT_OK;
 }
 

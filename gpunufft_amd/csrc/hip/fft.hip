@@ -1,0 +1,692 @@
+// Oversampled-grid FFT with the stages around it fused, for grid lengths
+// whose prime factors are 2, 3 and 5 (other lengths use rocFFT).
+//
+// The reference runs, per coil, INVERSE fftshift -> cuFFT -> INVERSE fftshift
+// -> crop -> 1/sqrt(Ng) -> deapodization -> conj(S) coil sum for the adjoint
+// and S -> deapodization -> zero pad -> INVERSE fftshift -> cuFFT ->
+// FORWARD fftshift for the forward (CUDA/src/gpuNUFFT_operator.cpp:682-850,
+// :1143-1261; CUDA/src/gpu/std_gpuNUFFT_kernels.cu:114-218, :341-372,
+// :493-521, :569-654, :699-793), each a separate pass over the whole grid.
+//
+// Here a d-dimensional transform is d passes, one per axis. A pass loads a
+// tile of lines into LDS, transforms them with a Stockham radix-8/4/2/3/5
+// FFT and writes them back. The passes are pruned to the data that matters:
+//   adjoint: the z and y passes write only the cropped image range of their
+//     axis; the last (x) pass writes image pixels and applies the shift,
+//     crop, 1/sqrt(Ng), deapodization and the conj(S) coil sum on the way out;
+//   forward: the first (x) pass reads image lines (S * x * deapo, zero
+//     padding and shift fused into the load) and the y and z passes read only
+//     the image range of their axis into otherwise zero lines.
+// The shifts are index maps: the adjoint's grid arrives INVERSE-shifted from
+// the fold; the forward's grid leaves as the plain spectrum and interp
+// applies the output shift in its own indices.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gnufft_internal.hpp"
+
+namespace gnufft {
+
+namespace {
+
+constexpr int kFftThreads = 256;
+constexpr int kTileElems = 8192;  // complex elements per LDS tile (64 KB)
+constexpr int kMaxStripe = 32;    // lines per tile of a y / z pass (256 B rows)
+constexpr int kMaxPix = 8;        // image pixels per thread in the x passes
+constexpr int kMaxLen = kMaxPix * kFftThreads;  // longest transform (2048)
+
+// ---------------------------------------------------------------------------
+// Small DFTs with kernel exp(sign * 2 pi i / R)
+// ---------------------------------------------------------------------------
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// sign * i * a
+__device__ inline float2 mul_si(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
+
+template <int R>
+__device__ inline void dft(float2* v, float sign);
+
+template <>
+__device__ inline void dft<2>(float2* v, float) {
+  const float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <>
+__device__ inline void dft<4>(float2* v, float sign) {
+  const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  const float2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sign);
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+template <>
+__device__ inline void dft<8>(float2* v, float sign) {
+  // radix-4 DFTs of the even and odd inputs, combined with w8^k
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e, sign);
+  dft<4>(o, sign);
+  const float h = 0.70710678118654752f;
+  // w8 = (1 + sign i)/sqrt2, w8^2 = sign i, w8^3 = (-1 + sign i)/sqrt2
+  const float2 o1 = make_float2(h * (o[1].x - sign * o[1].y), h * (o[1].y + sign * o[1].x));
+  const float2 o2 = mul_si(o[2], sign);
+  const float2 o3 = make_float2(h * (-o[3].x - sign * o[3].y), h * (-o[3].y + sign * o[3].x));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <>
+__device__ inline void dft<3>(float2* v, float sign) {
+  const float s = 0.86602540378443865f * sign;  // sin(2 pi / 3)
+  const float2 a = v[0];
+  const float2 t = cadd(v[1], v[2]);
+  const float2 u = csub(v[1], v[2]);
+  const float2 m = make_float2(a.x - 0.5f * t.x, a.y - 0.5f * t.y);
+  v[0] = cadd(a, t);
+  v[1] = make_float2(m.x - s * u.y, m.y + s * u.x);
+  v[2] = make_float2(m.x + s * u.y, m.y - s * u.x);
+}
+
+template <>
+__device__ inline void dft<5>(float2* v, float sign) {
+  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;  // cos 72, cos 144
+  const float s1 = 0.95105651629515357f * sign, s2 = 0.58778525229247313f * sign;
+  const float2 a = v[0];
+  const float2 t1 = cadd(v[1], v[4]), u1 = csub(v[1], v[4]);
+  const float2 t2 = cadd(v[2], v[3]), u2 = csub(v[2], v[3]);
+  const float2 m1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+  const float2 m2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+  const float2 n1 = make_float2(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
+  const float2 n2 = make_float2(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
+  v[0] = cadd(a, cadd(t1, t2));
+  v[1] = cadd(m1, n1);
+  v[4] = csub(m1, n1);
+  v[2] = cadd(m2, n2);
+  v[3] = csub(m2, n2);
+}
+
+// n / d for n * d < 2^32 via a multiply-high with m = ceil(2^32 / d); d == 1
+// has no 32-bit m and is passed as m == 0.
+__device__ inline int qdiv(int n, uint32_t m) {
+  return m ? (int)__umulhi((uint32_t)n, m) : n;
+}
+
+// One Stockham stage of a length-L transform (Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors", SC08):
+// butterfly j of a line reads elements j + s L/R, multiplies them by
+// w^(k s L/(Ns R)) with k = j mod Ns, runs a radix-R DFT and writes
+// elements (j / Ns) Ns R + k + s Ns.
+struct FftStage {
+  int R, Ns, per_line, tws;
+  uint32_t m_pl, m_ns;  // qdiv constants for per_line and Ns
+};
+
+struct FftRadices {
+  int n;
+  FftStage st[12];
+};
+
+template <int R>
+__device__ inline void stockham_stage(float2* buf, int lines, int LP, const FftStage& st,
+                                      const float2* tw, float sign) {
+  constexpr int KB = (kTileElems / kFftThreads + R - 1) / R;  // butterflies per thread
+  const int B = lines * st.per_line;
+  float2 v[KB][R];
+#pragma unroll
+  for (int q = 0; q < KB; q++) {
+    const int b = threadIdx.x + q * kFftThreads;
+    if (b < B) {
+      const int t = qdiv(b, st.m_pl);
+      const int j = b - t * st.per_line;
+      const float2* src = buf + t * LP + j;
+#pragma unroll
+      for (int s = 0; s < R; s++) v[q][s] = src[s * st.per_line];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < KB; q++) {
+    const int b = threadIdx.x + q * kFftThreads;
+    if (b < B) {
+      const int t = qdiv(b, st.m_pl);
+      const int j = b - t * st.per_line;
+      const int jq = qdiv(j, st.m_ns);
+      const int k = j - jq * st.Ns;
+      if (k) {
+        // k s tws < Ns R tws = L: the index needs no wrap
+#pragma unroll
+        for (int s = 1; s < R; s++) v[q][s] = cmul(v[q][s], tw[k * s * st.tws]);
+      }
+      dft<R>(v[q], sign);
+      float2* dst = buf + t * LP + jq * st.Ns * R + k;
+#pragma unroll
+      for (int s = 0; s < R; s++) dst[s * st.Ns] = v[q][s];
+    }
+  }
+  __syncthreads();
+}
+
+// Transform `lines` lines held in LDS (stride LP) in place. Callers
+// synchronise after filling the lines; the last stage ends with a barrier.
+__device__ inline void fft_lines(float2* buf, int lines, int LP, const FftRadices& rad,
+                                 const float2* tw, float sign) {
+  for (int i = 0; i < rad.n; i++) {
+    const FftStage& st = rad.st[i];
+    switch (st.R) {
+      case 8: stockham_stage<8>(buf, lines, LP, st, tw, sign); break;
+      case 4: stockham_stage<4>(buf, lines, LP, st, tw, sign); break;
+      case 2: stockham_stage<2>(buf, lines, LP, st, tw, sign); break;
+      case 3: stockham_stage<3>(buf, lines, LP, st, tw, sign); break;
+      default: stockham_stage<5>(buf, lines, LP, st, tw, sign); break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y / z pass over [coil][d2][d1][d0] arrays. Lines run along the pass axis;
+// a tile is `stripe` lines adjacent in x (a power of two, up to 32, that
+// fits the LDS tile) at one position of the remaining axis, so every global
+// row access covers `stripe` consecutive elements.
+// Input extent in_len (element i placed at wrap(i + in_off, L) of a zero
+// line when in_scatter), output extent out_len (element o taken from
+// wrap(o + out_off, L) when out_gather).
+// ---------------------------------------------------------------------------
+struct AxisPass {
+  const float2* in;
+  float2* out;
+  long long in_coil, out_coil;  // elements per coil
+  long long in_step, out_step;  // stride along the pass axis
+  long long in_ob, out_ob;      // stride along the remaining axis
+  int nx;                       // extent in x
+  int nob;                      // extent of the remaining axis
+  int L, LP;
+  int in_len, out_len;
+  int in_scatter, in_off;
+  int out_gather, out_off;
+  int stripe, lg_stripe, x_tiles;
+  float sign;
+  FftRadices rad;
+  const float2* tw;
+};
+
+__global__ void __launch_bounds__(kFftThreads) fft_axis_kernel(AxisPass p) {
+  extern __shared__ float2 lds[];
+  float2* tw = lds;
+  float2* buf = lds + p.L;
+  // blockIdx.x = (coil * nob + ob) * x_tiles + xt
+  const int xt = blockIdx.x % p.x_tiles;
+  const int r = blockIdx.x / p.x_tiles;
+  const int ob = r % p.nob;
+  const int coil = r / p.nob;
+  const int x0 = xt * p.stripe;
+  const int nl = min(p.stripe, p.nx - x0);
+  for (int i = threadIdx.x; i < p.L; i += kFftThreads) tw[i] = p.tw[i];
+  const float2* in = p.in + coil * p.in_coil + ob * p.in_ob + x0;
+  float2* out = p.out + coil * p.out_coil + ob * p.out_ob + x0;
+  if (p.in_scatter) {
+    for (int e = threadIdx.x; e < nl * p.LP; e += kFftThreads) buf[e] = make_float2(0.f, 0.f);
+    __syncthreads();
+  }
+  const int n_in = p.in_len << p.lg_stripe;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < n_in; e += kFftThreads) {
+    const int l = e & (p.stripe - 1), i = e >> p.lg_stripe;
+    if (l < nl) {
+      const int pos = p.in_scatter ? wrap(i + p.in_off, p.L) : i;
+      buf[l * p.LP + pos] = in[(long long)i * p.in_step + l];
+    }
+  }
+  __syncthreads();
+  fft_lines(buf, nl, p.LP, p.rad, tw, p.sign);
+  const int n_out = p.out_len << p.lg_stripe;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < n_out; e += kFftThreads) {
+    const int l = e & (p.stripe - 1), o = e >> p.lg_stripe;
+    if (l < nl) {
+      const int src = p.out_gather ? wrap(o + p.out_off, p.L) : o;
+      out[(long long)o * p.out_step + l] = buf[l * p.LP + src];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint last pass (x): lines [coil][N2][N1][G0]; pixel x of an image line
+// reads FFT index wrap(x + crop + shift_inv, G0), scaled by 1/sqrt(Ng)
+// [* deapo], then either summed over coils with conj(S) or written per coil
+// (CUDA/src/gpu/std_gpuNUFFT_kernels.cu:114-218, :341-372, :493-521).
+// A workgroup takes Q image lines and walks the coils in groups of Cg.
+// ---------------------------------------------------------------------------
+struct LastPass {
+  const float2* in;
+  float2* out;  // [img] with sens, else [coil][img]
+  const float* deapo;
+  const float2* sens;  // null: per-coil output
+  int G0, N0, LP;
+  int lines;  // N1 * N2
+  int crop0, shift0;
+  int coils, Cg, Q;
+  uint32_t m_g0, m_n0;  // qdiv constants for G0, N0
+  float scale;
+  int apply_deapo, accumulate;
+  float sign;
+  FftRadices rad;
+  const float2* tw;
+};
+
+__global__ void __launch_bounds__(kFftThreads) fft_adjoint_last_kernel(LastPass p) {
+  extern __shared__ float2 lds[];
+  float2* tw = lds;
+  float2* buf = lds + p.G0;
+  const int l0 = blockIdx.x * p.Q;
+  const int nq = min(p.Q, p.lines - l0);
+  for (int i = threadIdx.x; i < p.G0; i += kFftThreads) tw[i] = p.tw[i];
+  const long long Ni = (long long)p.N0 * p.lines;
+  const long long cin = (long long)p.G0 * p.lines;
+  const long long pix0 = (long long)l0 * p.N0;
+  const int npix = nq * p.N0;
+  float2 acc[kMaxPix];
+#pragma unroll
+  for (int k = 0; k < kMaxPix; k++) {
+    acc[k] = make_float2(0.f, 0.f);
+    const int e = threadIdx.x + k * kFftThreads;
+    if (p.sens && p.accumulate && e < npix) acc[k] = p.out[pix0 + e];
+  }
+  const int per_coil = nq * p.G0;  // the Q lines of one coil are contiguous
+  for (int c0 = 0; c0 < p.coils; c0 += p.Cg) {
+    const int ng = min(p.Cg, p.coils - c0);
+    __syncthreads();  // the previous group's reads of buf are done
+    for (int c = 0; c < ng; c++) {
+      const float2* src = p.in + (long long)(c0 + c) * cin + (long long)l0 * p.G0;
+#pragma unroll 4
+      for (int e = threadIdx.x; e < per_coil; e += kFftThreads) {
+        const int q = qdiv(e, p.m_g0);
+        const int x = e - q * p.G0;
+        buf[(c * nq + q) * p.LP + x] = src[e];
+      }
+    }
+    __syncthreads();
+    fft_lines(buf, ng * nq, p.LP, p.rad, tw, p.sign);
+#pragma unroll
+    for (int k = 0; k < kMaxPix; k++) {
+      const int e = threadIdx.x + k * kFftThreads;
+      if (e >= npix) continue;
+      const int q = qdiv(e, p.m_n0);
+      const int x = e - q * p.N0;
+      const long long pix = pix0 + e;
+      const int m = wrap(x + p.crop0 + p.shift0, p.G0);
+      const float d = p.apply_deapo ? p.deapo[pix] : 1.0f;
+      for (int c = 0; c < ng; c++) {
+        float2 v = buf[(c * nq + q) * p.LP + m];
+        v.x = (v.x * p.scale) * d;
+        v.y = (v.y * p.scale) * d;
+        if (p.sens) {
+          const float2 s = p.sens[(long long)(c0 + c) * Ni + pix];
+          acc[k].x += v.x * s.x + v.y * s.y;
+          acc[k].y += v.y * s.x - v.x * s.y;
+        } else {
+          p.out[(long long)(c0 + c) * Ni + pix] = v;
+        }
+      }
+    }
+  }
+  if (p.sens) {
+#pragma unroll
+    for (int k = 0; k < kMaxPix; k++) {
+      const int e = threadIdx.x + k * kFftThreads;
+      if (e < npix) p.out[pix0 + e] = acc[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward first pass (x): image lines -> [coil][N2][N1][G0]. Pixel x goes to
+// position wrap(x + crop - shift_inv, G0) of a zero line (zero padding and
+// INVERSE fftshift) with value (S_c * x) * deapo
+// (CUDA/src/gpu/std_gpuNUFFT_kernels.cu:142-218, :493-521, :699-793).
+// ---------------------------------------------------------------------------
+struct FirstPass {
+  const float2* img;  // [img] (sens) or [coil][img]
+  float2* out;
+  const float* deapo;
+  const float2* sens;
+  int img_per_coil;
+  int G0, N0, LP;
+  int lines;
+  int crop0, shift0;
+  int coils, Cg, Q;
+  uint32_t m_g0, m_n0;
+  float sign;
+  FftRadices rad;
+  const float2* tw;
+};
+
+__global__ void __launch_bounds__(kFftThreads) fft_forward_first_kernel(FirstPass p) {
+  extern __shared__ float2 lds[];
+  float2* tw = lds;
+  float2* buf = lds + p.G0;
+  const int l0 = blockIdx.x * p.Q;
+  const int nq = min(p.Q, p.lines - l0);
+  for (int i = threadIdx.x; i < p.G0; i += kFftThreads) tw[i] = p.tw[i];
+  const long long Ni = (long long)p.N0 * p.lines;
+  const long long cout = (long long)p.G0 * p.lines;
+  const long long pix0 = (long long)l0 * p.N0;
+  const int npix = nq * p.N0;
+  const int per_coil = nq * p.G0;
+  for (int c0 = 0; c0 < p.coils; c0 += p.Cg) {
+    const int ng = min(p.Cg, p.coils - c0);
+    __syncthreads();  // the previous group's reads of buf are done
+    for (int e = threadIdx.x; e < ng * nq * p.LP; e += kFftThreads) buf[e] = make_float2(0.f, 0.f);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kMaxPix; k++) {
+      const int e = threadIdx.x + k * kFftThreads;
+      if (e >= npix) continue;
+      const int q = qdiv(e, p.m_n0);
+      const int x = e - q * p.N0;
+      const long long pix = pix0 + e;
+      const float d = p.deapo[pix];
+      const int pos = wrap(x + p.crop0 - p.shift0, p.G0);
+      const float2 shared_img = p.img_per_coil ? make_float2(0.f, 0.f) : p.img[pix];
+      for (int c = 0; c < ng; c++) {
+        float2 v = p.img_per_coil ? p.img[(long long)(c0 + c) * Ni + pix] : shared_img;
+        if (p.sens) v = cmul(v, p.sens[(long long)(c0 + c) * Ni + pix]);
+        buf[(c * nq + q) * p.LP + pos] = make_float2(v.x * d, v.y * d);
+      }
+    }
+    __syncthreads();
+    fft_lines(buf, ng * nq, p.LP, p.rad, tw, p.sign);
+    for (int c = 0; c < ng; c++) {
+      float2* dst = p.out + (long long)(c0 + c) * cout + (long long)l0 * p.G0;
+#pragma unroll 4
+      for (int e = threadIdx.x; e < per_coil; e += kFftThreads) {
+        const int q = qdiv(e, p.m_g0);
+        const int x = e - q * p.G0;
+        dst[e] = buf[(c * nq + q) * p.LP + x];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+uint32_t qdiv_const(int d) { return d == 1 ? 0u : udiv_magic((uint32_t)d); }
+
+bool make_radices(int L, FftRadices* rad) {
+  rad->n = 0;
+  int n = L;
+  const int order[5] = {8, 4, 2, 3, 5};
+  for (int r : order)
+    while (n > 1 && n % r == 0) {
+      if (rad->n == 12) return false;
+      rad->st[rad->n++].R = r;
+      n /= r;
+    }
+  if (n != 1 || rad->n == 0) return false;
+  int Ns = 1;
+  for (int i = 0; i < rad->n; i++) {
+    FftStage& s = rad->st[i];
+    s.Ns = Ns;
+    s.per_line = L / s.R;
+    s.tws = L / (Ns * s.R);
+    s.m_pl = qdiv_const(s.per_line);
+    s.m_ns = qdiv_const(Ns);
+    Ns *= s.R;
+  }
+  return true;
+}
+
+int pad_of(int L) { return L + 1; }
+
+size_t tile_bytes(int L, int lines) {
+  return ((size_t)L + (size_t)lines * pad_of(L)) * sizeof(float2);
+}
+
+template <typename K>
+hipError_t allow_lds(K kernel, size_t lds) {
+  if (lds <= 64 * 1024) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+// x-pass tile: Cg coils x Q lines with Cg * Q * LP <= kTileElems and
+// Q * N0 <= kMaxPix * kFftThreads (N0 <= G0 <= kMaxLen keeps Q >= 1 inside
+// both bounds).
+void x_tiling(int G0, int N0, int coils, int* Cg, int* Q) {
+  const int per_tile = std::max(1, kTileElems / pad_of(G0));
+  *Cg = std::max(1, std::min(coils, per_tile));
+  *Q = std::max(1, std::min(per_tile / *Cg, (kMaxPix * kFftThreads) / N0));
+}
+
+hipError_t launch_axis(AxisPass p, int coils, hipStream_t s) {
+  p.LP = pad_of(p.L);
+  if (!make_radices(p.L, &p.rad)) return hipErrorInvalidValue;
+  p.lg_stripe = 0;
+  while ((2 << p.lg_stripe) <= kMaxStripe && (2 << p.lg_stripe) * p.LP <= kTileElems) p.lg_stripe++;
+  p.stripe = 1 << p.lg_stripe;
+  p.x_tiles = (p.nx + p.stripe - 1) / p.stripe;
+  const size_t lds = tile_bytes(p.L, p.stripe);
+  hipError_t e = allow_lds(fft_axis_kernel, lds);
+  if (e != hipSuccess) return e;
+  const unsigned blocks = (unsigned)((long long)coils * p.nob * p.x_tiles);
+  hipLaunchKernelGGL(fft_axis_kernel, dim3(blocks), dim3(kFftThreads), lds, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool fused_fft_supported(const Geom& g) {
+  FftRadices r;
+  for (int a = 0; a < g.dim; a++) {
+    // one padded line fits a tile; the x tiling needs N0 <= G0 <= kMaxLen
+    if (g.G[a] > kMaxLen || !make_radices(g.G[a], &r)) return false;
+    if (g.N[a] > g.G[a]) return false;
+  }
+  return true;
+}
+
+size_t fused_fft_tmp_elems(const Geom& g) {
+  // adjoint z-pass / forward y-pass output (3-D), x-pass output (2-D)
+  return g.dim == 3 ? (size_t)g.G[0] * g.G[1] * g.N[2] : (size_t)g.G[0] * g.N[1];
+}
+
+// exp(sign 2 pi i k / G_a) for k < G_a, computed in double, laid out
+// [axis][sign -1, +1][G_a]; offsets[a][0 | 1] index the table.
+std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][2]) {
+  std::vector<float2> t;
+  for (int a = 0; a < 3; a++) offsets[a][0] = offsets[a][1] = 0;
+  for (int a = 0; a < g.dim; a++)
+    for (int si = 0; si < 2; si++) {
+      offsets[a][si] = t.size();
+      const double sign = si == 0 ? -1.0 : 1.0;
+      const int L = g.G[a];
+      for (int k = 0; k < L; k++) {
+        const double ang = sign * 2.0 * M_PI * (double)k / (double)L;
+        t.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+      }
+    }
+  return t;
+}
+
+// Adjoint: grid [coils][G2][G1][G0] (the INVERSE-shifted spectrum) -> image.
+// tmp holds coils * fused_fft_tmp_elems(g) elements; the grid is
+// overwritten. stop == 1 (FFT): per-coil images scaled by 1/sqrt(Ng) only.
+// accumulate: add the coil sum to `out` (later coil batches).
+hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,
+                             const float* deapo, const float2* sens, float2* out, int stop,
+                             bool accumulate, const float2* tw, const size_t tw_off[3][2],
+                             hipStream_t s) {
+  const float sign = 1.0f;  // unnormalised inverse transform
+  const int G0 = g.G[0], G1 = g.G[1];
+  const int N1 = g.N[1], N2 = g.dim == 3 ? g.N[2] : 1;
+  const float2* cur = grid;
+  if (g.dim == 3) {
+    // z: [c][G2][G1][G0] -> tmp [c][N2][G1][G0]
+    AxisPass z{};
+    z.in = grid;
+    z.out = tmp;
+    z.in_coil = (long long)G0 * G1 * g.G[2];
+    z.out_coil = (long long)G0 * G1 * N2;
+    z.in_step = z.out_step = (long long)G0 * G1;
+    z.in_ob = z.out_ob = G0;
+    z.nx = G0;
+    z.nob = G1;
+    z.L = g.G[2];
+    z.in_len = g.G[2];
+    z.out_len = N2;
+    z.out_gather = 1;
+    z.out_off = g.crop[2] + g.shift_inv[2];
+    z.sign = sign;
+    z.tw = tw + tw_off[2][1];
+    hipError_t e = launch_axis(z, coils, s);
+    if (e != hipSuccess) return e;
+    cur = tmp;
+  }
+  // y: [c][N2][G1][G0] -> [c][N2][N1][G0] (3-D: tmp -> grid, 2-D: grid -> tmp)
+  float2* ydst = g.dim == 3 ? grid : tmp;
+  {
+    AxisPass y{};
+    y.in = cur;
+    y.out = ydst;
+    y.in_coil = (long long)G0 * G1 * N2;
+    y.out_coil = (long long)G0 * N1 * N2;
+    y.in_step = y.out_step = G0;
+    y.in_ob = (long long)G0 * G1;
+    y.out_ob = (long long)G0 * N1;
+    y.nx = G0;
+    y.nob = N2;
+    y.L = G1;
+    y.in_len = G1;
+    y.out_len = N1;
+    y.out_gather = 1;
+    y.out_off = g.crop[1] + g.shift_inv[1];
+    y.sign = sign;
+    y.tw = tw + tw_off[1][1];
+    hipError_t e = launch_axis(y, coils, s);
+    if (e != hipSuccess) return e;
+  }
+  LastPass p{};
+  p.in = ydst;
+  p.out = out;
+  p.deapo = deapo;
+  p.sens = stop == 1 ? nullptr : sens;
+  p.G0 = G0;
+  p.N0 = g.N[0];
+  p.LP = pad_of(G0);
+  p.lines = N1 * N2;
+  p.crop0 = g.crop[0];
+  p.shift0 = g.shift_inv[0];
+  p.coils = coils;
+  x_tiling(G0, p.N0, coils, &p.Cg, &p.Q);
+  p.m_g0 = qdiv_const(G0);
+  p.m_n0 = qdiv_const(p.N0);
+  p.scale = g.fft_scale;
+  p.apply_deapo = stop == 1 ? 0 : 1;
+  p.accumulate = accumulate ? 1 : 0;
+  p.sign = sign;
+  if (!make_radices(G0, &p.rad)) return hipErrorInvalidValue;
+  p.tw = tw + tw_off[0][1];
+  const size_t lds = tile_bytes(G0, p.Q * p.Cg);
+  hipError_t e = allow_lds(fft_adjoint_last_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fft_adjoint_last_kernel, dim3((unsigned)((p.lines + p.Q - 1) / p.Q)),
+                     dim3(kFftThreads), lds, s, p);
+  return hipGetLastError();
+}
+
+// Forward: image -> grid [coils][G2][G1][G0], the spectrum of the
+// zero-padded, INVERSE-shifted, deapodized (and coil-weighted) image. tmp
+// holds coils * fused_fft_tmp_elems(g) elements.
+hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil,
+                             const float2* sens, const float* deapo, float2* grid, float2* tmp,
+                             int coils, const float2* tw, const size_t tw_off[3][2],
+                             hipStream_t s) {
+  const float sign = -1.0f;
+  const int G0 = g.G[0], G1 = g.G[1];
+  const int N1 = g.N[1], N2 = g.dim == 3 ? g.N[2] : 1;
+  FirstPass f{};
+  f.img = img;
+  f.deapo = deapo;
+  f.sens = sens;
+  f.img_per_coil = img_per_coil ? 1 : 0;
+  f.G0 = G0;
+  f.N0 = g.N[0];
+  f.LP = pad_of(G0);
+  f.lines = N1 * N2;
+  f.crop0 = g.crop[0];
+  f.shift0 = g.shift_inv[0];
+  f.coils = coils;
+  x_tiling(G0, f.N0, coils, &f.Cg, &f.Q);
+  f.m_g0 = qdiv_const(G0);
+  f.m_n0 = qdiv_const(f.N0);
+  f.sign = sign;
+  if (!make_radices(G0, &f.rad)) return hipErrorInvalidValue;
+  f.tw = tw + tw_off[0][0];
+  // 3-D: x -> grid (as [c][N2][N1][G0]), y -> tmp, z -> grid
+  // 2-D: x -> tmp, y -> grid
+  f.out = g.dim == 3 ? grid : tmp;
+  const size_t lds = tile_bytes(G0, f.Q * f.Cg);
+  hipError_t e = allow_lds(fft_forward_first_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fft_forward_first_kernel, dim3((unsigned)((f.lines + f.Q - 1) / f.Q)),
+                     dim3(kFftThreads), lds, s, f);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // y: [c][N2][N1][G0] -> [c][N2][G1][G0]
+  AxisPass y{};
+  y.in = f.out;
+  y.out = g.dim == 3 ? tmp : grid;
+  y.in_coil = (long long)G0 * N1 * N2;
+  y.out_coil = (long long)G0 * G1 * N2;
+  y.in_step = y.out_step = G0;
+  y.in_ob = (long long)G0 * N1;
+  y.out_ob = (long long)G0 * G1;
+  y.nx = G0;
+  y.nob = N2;
+  y.L = G1;
+  y.in_len = N1;
+  y.out_len = G1;
+  y.in_scatter = 1;
+  y.in_off = g.crop[1] - g.shift_inv[1];
+  y.sign = sign;
+  y.tw = tw + tw_off[1][0];
+  e = launch_axis(y, coils, s);
+  if (e != hipSuccess || g.dim == 2) return e;
+  // z: [c][N2][G1][G0] -> [c][G2][G1][G0]
+  AxisPass z{};
+  z.in = tmp;
+  z.out = grid;
+  z.in_coil = (long long)G0 * G1 * N2;
+  z.out_coil = (long long)G0 * G1 * g.G[2];
+  z.in_step = z.out_step = (long long)G0 * G1;
+  z.in_ob = z.out_ob = G0;
+  z.nx = G0;
+  z.nob = G1;
+  z.L = g.G[2];
+  z.in_len = N2;
+  z.out_len = g.G[2];
+  z.in_scatter = 1;
+  z.in_off = g.crop[2] - g.shift_inv[2];
+  z.sign = sign;
+  z.tw = tw + tw_off[2][0];
+  return launch_axis(z, coils, s);
+}
+
+}  // namespace gnufft

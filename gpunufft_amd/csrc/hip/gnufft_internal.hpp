@@ -310,4 +310,17 @@ int fold_threads(const Geom& g, const Cover& cover);
 hipError_t launch_post_batch(const BatchOp* ops, const BatchList& l, hipStream_t s);
 hipError_t launch_pre_batch(const BatchOp* ops, const BatchList& l, hipStream_t s);
 
+// Fused oversampled-grid FFT (fft.hip).
+bool fused_fft_supported(const Geom& g);
+size_t fused_fft_tmp_elems(const Geom& g);
+std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][2]);
+hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,
+                             const float* deapo, const float2* sens, float2* out, int stop,
+                             bool accumulate, const float2* tw, const size_t tw_off[3][2],
+                             hipStream_t s);
+hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil,
+                             const float2* sens, const float* deapo, float2* grid, float2* tmp,
+                             int coils, const float2* tw, const size_t tw_off[3][2],
+                             hipStream_t s);
+
 }  // namespace gnufft

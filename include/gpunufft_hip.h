@@ -107,6 +107,18 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
 int gnufft_plan_destroy(gnufft_plan plan);
 int gnufft_plan_get_info(gnufft_plan plan, gnufft_plan_info* info);
 
+/* Oversampled-grid FFT implementation of a plan. The reference runs cuFFT
+ * plus separate shift / scale / crop / pad / deapodization / coil kernels
+ * (CUDA/src/gpuNUFFT_operator.cpp:682-850, :1143-1261).
+ * GNUFFT_FFT_FUSED: one pass per axis, pruned to the image range, with those
+ *   stages fused into the first and last pass (grid lengths 2^a 3^b 5^c up
+ *   to 2048; the default where supported).
+ * GNUFFT_FFT_ROCFFT: rocFFT plus the fused pre / post kernels (any length).
+ * set_fft returns GNUFFT_ERR_UNSUPPORTED for FUSED on other grid lengths. */
+enum { GNUFFT_FFT_FUSED = 0, GNUFFT_FFT_ROCFFT = 1 };
+int gnufft_plan_set_fft(gnufft_plan plan, int kind);
+int gnufft_plan_get_fft(gnufft_plan plan, int* kind);
+
 /* Sector precompute on the device: assignSectors + sort + sectorDataCount +
  * sorted trajectory/density (CUDA/src/gpuNUFFT_operator_factory.cpp:128-205,
  * :450-553; CUDA/src/gpu/precomp_kernels.cu:8-177). traj/dens live at

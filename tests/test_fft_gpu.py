@@ -1,0 +1,140 @@
+"""Fused oversampled-grid FFT (gnufft_plan_set_fft FUSED) against the rocFFT
+path and the oracle.
+
+The fused path replaces shift -> FFT -> shift -> crop -> scale ->
+deapodization -> coil sum (adjoint) and coil weight -> deapodization -> pad
+-> shift -> FFT (forward) by one pruned pass per axis. Each case runs the
+full adjoint (stops FFT and DEAPODIZATION, with and without sensitivities)
+and forward on one plan with both FFT kinds; both must equal the oracle
+(rel-L2 <= 1e-5) and each other. The grid lengths cover even and odd
+lengths, radix 2/3/4/5/8 mixes, anisotropic 3-D grids and a length with a
+factor 7 (rocFFT only: set_fft(FUSED) is refused).
+"""
+
+import numpy as np
+import pytest
+
+from gpunufft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+# (image dims, osf) -> grid lengths
+CASES = [
+    ((32, 32), 1.5),        # 48 = 8 * 2 * 3
+    ((30, 30), 1.5),        # 45 = 3 * 3 * 5 (odd)
+    ((50, 40), 1.5),        # 75 x 60
+    ((64, 64), 2.0),        # 128
+    ((20, 24, 16), 1.5),    # 30 x 36 x 24
+    ((16, 16, 16), 2.0),    # 32^3
+    ((25, 16, 10), 2.0),    # 50 x 32 x 20
+]
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a ROCm device"
+    from gpunufft_amd import capi
+    capi.load()
+    return torch, capi
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _traj(dims, seed):
+    rng = np.random.default_rng(seed)
+    d = len(dims)
+    return rng.uniform(-0.5, 0.5, (d, 700 if d == 2 else 1500)).astype(np.float32)
+
+
+@pytest.mark.parametrize("dims,osf", CASES, ids=[f"{'x'.join(map(str, d))}_osf{o}" for d, o in CASES])
+@pytest.mark.parametrize("with_sens", [False, True], ids=["nosens", "sens"])
+def test_fused_equals_rocfft_and_oracle(oracle, env, dims, osf, with_sens):
+    torch, capi = env
+    C = 3
+    traj = _traj(dims, 1)
+    dens = W.ramp_density(traj, len(dims) - 1)
+    sens = W.gaussian_sens(dims, C, seed=4) if with_sens else None
+    rng = np.random.default_rng(7)
+    kd = W.complex_normal(rng, (C, traj.shape[1]))
+    ishape = tuple(dims[::-1])
+    img = W.complex_normal(rng, ((1,) if with_sens else (C,)) + ishape)
+    op = oracle.Operator(traj, dims, 3, 8, osf, dens=dens, sens=sens, order="stable")
+    p = capi.Plan(dims, 3, 8, osf)
+    p.set_trajectory(_dev(torch, traj), _dev(torch, dens))
+    if with_sens:
+        p.set_sens(_dev(torch, sens), C)
+    assert p.fft() == capi.FFT_FUSED
+    print("case", dims, osf, "sens" if with_sens else "nosens", flush=True)
+    results = {}
+    for kind in (capi.FFT_FUSED, capi.FFT_ROCFFT):
+        p.set_fft(kind)
+        a_img = torch.empty(((1,) if with_sens else (C,)) + ishape, dtype=torch.complex64,
+                            device="cuda")
+        a_fft = torch.empty((C,) + ishape, dtype=torch.complex64, device="cuda")
+        f_out = torch.empty((C, traj.shape[1]), dtype=torch.complex64, device="cuda")
+        p.adjoint(_dev(torch, kd), a_img, C, stop=capi.DEAPODIZATION)
+        p.adjoint(_dev(torch, kd), a_fft, C, stop=capi.FFT)
+        p.forward(_dev(torch, img), f_out, C)
+        torch.cuda.synchronize()
+        results[kind] = (a_img.cpu().numpy(), a_fft.cpu().numpy(), f_out.cpu().numpy())
+    ref = (op.adjoint(kd), op.adjoint(kd, stop=oracle.FFT), op.forward(img, C))
+    for kind, got in results.items():
+        for name, g, r in zip(("adjoint", "adjoint_fft", "forward"), got, ref):
+            err = oracle.rel_l2(g, r)
+            assert err < TOL, (kind, name, err)
+    for j, name in enumerate(("adjoint", "adjoint_fft", "forward")):
+        assert oracle.rel_l2(results[capi.FFT_FUSED][j], results[capi.FFT_ROCFFT][j]) < TOL, name
+
+
+def test_unsupported_length_uses_rocfft(oracle, env):
+    torch, capi = env
+    dims, osf = (28, 28), 1.5  # 42 = 2 * 3 * 7
+    p = capi.Plan(dims, 3, 8, osf)
+    assert p.fft() == capi.FFT_ROCFFT
+    with pytest.raises(capi.GnufftError) as e:
+        p.set_fft(capi.FFT_FUSED)
+    assert e.value.code == capi.ERR_UNSUPPORTED
+    traj = _traj(dims, 2)
+    p.set_trajectory(_dev(torch, traj))
+    rng = np.random.default_rng(3)
+    kd = W.complex_normal(rng, (2, traj.shape[1]))
+    out = torch.empty((2, 28, 28), dtype=torch.complex64, device="cuda")
+    p.adjoint(_dev(torch, kd), out, 2)
+    torch.cuda.synchronize()
+    op = oracle.Operator(traj, dims, 3, 8, osf, order="stable")
+    assert oracle.rel_l2(out.cpu().numpy(), op.adjoint(kd)) < TOL
+
+
+def test_fused_headline_geometry_matches_rocfft(env):
+    """koosh 128^3, osf 1.5 (192^3 grid), 8 coils with sensitivities: the
+    fused and rocFFT paths agree on the full adjoint and forward."""
+    torch, capi = env
+    dims = (128, 128, 128)
+    traj = W.koosh_3d(2048, 256)  # fewer spokes than the full workload
+    C = 8
+    sens = W.gaussian_sens(dims, C, seed=0)
+    p = capi.Plan(dims, 3, 8, 1.5)
+    p.set_trajectory(_dev(torch, traj), _dev(torch, W.ramp_density(traj, 2)))
+    p.set_sens(_dev(torch, sens), C)
+    rng = np.random.default_rng(5)
+    kd = _dev(torch, W.complex_normal(rng, (C, traj.shape[1])))
+    img = _dev(torch, W.complex_normal(rng, (1,) + tuple(dims[::-1])))
+    outs = {}
+    for kind in (capi.FFT_FUSED, capi.FFT_ROCFFT):
+        p.set_fft(kind)
+        a = torch.empty((1,) + tuple(dims[::-1]), dtype=torch.complex64, device="cuda")
+        f = torch.empty((C, traj.shape[1]), dtype=torch.complex64, device="cuda")
+        p.adjoint(kd, a, C)
+        p.forward(img, f, C)
+        torch.cuda.synchronize()
+        outs[kind] = (a.cpu().numpy(), f.cpu().numpy())
+
+    def rel(x, y):
+        return float(np.linalg.norm((x - y).ravel()) / np.linalg.norm(y.ravel()))
+
+    assert rel(outs[capi.FFT_FUSED][0], outs[capi.FFT_ROCFFT][0]) < TOL
+    assert rel(outs[capi.FFT_FUSED][1], outs[capi.FFT_ROCFFT][1]) < TOL
