@@ -33,10 +33,10 @@ namespace gnufft {
 
 namespace {
 
-constexpr int kFftThreads = 256;
+constexpr int kFftThreads = 512;
 constexpr int kTileElems = 8192;  // complex elements per LDS tile (64 KB)
 constexpr int kMaxStripe = 32;    // lines per tile of a y / z pass (256 B rows)
-constexpr int kMaxPix = 8;        // image pixels per thread in the x passes
+constexpr int kMaxPix = 4;        // image pixels per thread in the x passes
 constexpr int kMaxLen = kMaxPix * kFftThreads;  // longest transform (2048)
 
 // ---------------------------------------------------------------------------
@@ -128,14 +128,22 @@ __device__ inline int qdiv(int n, uint32_t m) {
   return m ? (int)__umulhi((uint32_t)n, m) : n;
 }
 
+__device__ inline uint32_t qdiv_const_dev(int d) {
+  return d == 1 ? 0u : (uint32_t)((0x100000000ull + (uint32_t)d - 1) / (uint32_t)d);
+}
+
 // One Stockham stage of a length-L transform (Govindaraju et al., "High
 // performance discrete Fourier transforms on graphics processors", SC08):
 // butterfly j of a line reads elements j + s L/R, multiplies them by
 // w^(k s L/(Ns R)) with k = j mod Ns, runs a radix-R DFT and writes
 // elements (j / Ns) Ns R + k + s Ns.
+//
+// Butterflies of a tile are numbered b = j * lines + l: the lanes of a wave
+// walk the lines at a fixed j, so the LDS addresses l * LP + f(j) of one
+// access (odd line pitch LP) fall into distinct banks in every stage.
 struct FftStage {
   int R, Ns, per_line, tws;
-  uint32_t m_pl, m_ns;  // qdiv constants for per_line and Ns
+  uint32_t m_ns;  // qdiv constant for Ns
 };
 
 struct FftRadices {
@@ -144,58 +152,80 @@ struct FftRadices {
 };
 
 template <int R>
-__device__ inline void stockham_stage(float2* buf, int lines, int LP, const FftStage& st,
-                                      const float2* tw, float sign) {
-  constexpr int KB = (kTileElems / kFftThreads + R - 1) / R;  // butterflies per thread
+struct Bfly {
+  static constexpr int KB = (kTileElems / kFftThreads + R - 1) / R;  // butterflies per thread
+};
+
+// Twiddle, radix-R DFT and store of butterfly j (values v) at
+// (j / Ns) Ns R + k + s Ns of line l, through `put(l, index, value)`.
+template <int R, typename Put>
+__device__ inline void finish_butterfly(float2* v, int j, int l, const FftStage& st,
+                                        const float2* tw, float sign, Put put) {
+  const int jq = qdiv(j, st.m_ns);
+  const int k = j - jq * st.Ns;
+  if (k) {
+    // k s tws < Ns R tws = L: the index needs no wrap
+#pragma unroll
+    for (int s = 1; s < R; s++) v[s] = cmul(v[s], tw[k * s * st.tws]);
+  }
+  dft<R>(v, sign);
+  const int base = jq * st.Ns * R + k;
+#pragma unroll
+  for (int s = 0; s < R; s++) put(l, base + s * st.Ns, v[s]);
+}
+
+// Stage with its inputs fetched by get(l, index) and outputs stored by
+// put(l, index, value); a barrier separates all fetches from all stores
+// when `sync` (in-place LDS stages).
+template <int R, typename Get, typename Put>
+__device__ inline void stage(int lines, uint32_t m_lines, const FftStage& st, const float2* tw,
+                             float sign, bool sync, Get get, Put put) {
+  constexpr int KB = Bfly<R>::KB;
   const int B = lines * st.per_line;
   float2 v[KB][R];
 #pragma unroll
   for (int q = 0; q < KB; q++) {
     const int b = threadIdx.x + q * kFftThreads;
     if (b < B) {
-      const int t = qdiv(b, st.m_pl);
-      const int j = b - t * st.per_line;
-      const float2* src = buf + t * LP + j;
+      const int j = qdiv(b, m_lines);
+      const int l = b - j * lines;
 #pragma unroll
-      for (int s = 0; s < R; s++) v[q][s] = src[s * st.per_line];
+      for (int s = 0; s < R; s++) v[q][s] = get(l, j + s * st.per_line);
     }
   }
-  __syncthreads();
+  if (sync) __syncthreads();
 #pragma unroll
   for (int q = 0; q < KB; q++) {
     const int b = threadIdx.x + q * kFftThreads;
     if (b < B) {
-      const int t = qdiv(b, st.m_pl);
-      const int j = b - t * st.per_line;
-      const int jq = qdiv(j, st.m_ns);
-      const int k = j - jq * st.Ns;
-      if (k) {
-        // k s tws < Ns R tws = L: the index needs no wrap
-#pragma unroll
-        for (int s = 1; s < R; s++) v[q][s] = cmul(v[q][s], tw[k * s * st.tws]);
-      }
-      dft<R>(v[q], sign);
-      float2* dst = buf + t * LP + jq * st.Ns * R + k;
-#pragma unroll
-      for (int s = 0; s < R; s++) dst[s * st.Ns] = v[q][s];
+      const int j = qdiv(b, m_lines);
+      const int l = b - j * lines;
+      finish_butterfly<R>(v[q], j, l, st, tw, sign, put);
     }
   }
-  __syncthreads();
+}
+
+template <typename Get, typename Put>
+__device__ inline void stage_any(int lines, uint32_t m_lines, const FftStage& st,
+                                 const float2* tw, float sign, bool sync, Get get, Put put) {
+  switch (st.R) {
+    case 8: stage<8>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 4: stage<4>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 2: stage<2>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 3: stage<3>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    default: stage<5>(lines, m_lines, st, tw, sign, sync, get, put); break;
+  }
 }
 
 // Transform `lines` lines held in LDS (stride LP) in place. Callers
-// synchronise after filling the lines; the last stage ends with a barrier.
-__device__ inline void fft_lines(float2* buf, int lines, int LP, const FftRadices& rad,
-                                 const float2* tw, float sign) {
+// synchronise after filling the lines; every stage ends with a barrier.
+__device__ inline void fft_lines(float2* buf, int lines, uint32_t m_lines, int LP,
+                                 const FftRadices& rad, const float2* tw, float sign) {
+  auto get = [&](int l, int i) { return buf[l * LP + i]; };
+  auto put = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
   for (int i = 0; i < rad.n; i++) {
-    const FftStage& st = rad.st[i];
-    switch (st.R) {
-      case 8: stockham_stage<8>(buf, lines, LP, st, tw, sign); break;
-      case 4: stockham_stage<4>(buf, lines, LP, st, tw, sign); break;
-      case 2: stockham_stage<2>(buf, lines, LP, st, tw, sign); break;
-      case 3: stockham_stage<3>(buf, lines, LP, st, tw, sign); break;
-      default: stockham_stage<5>(buf, lines, LP, st, tw, sign); break;
-    }
+    stage_any(lines, m_lines, rad.st[i], tw, sign, true, get, put);
+    __syncthreads();
   }
 }
 
@@ -203,10 +233,12 @@ __device__ inline void fft_lines(float2* buf, int lines, int LP, const FftRadice
 // y / z pass over [coil][d2][d1][d0] arrays. Lines run along the pass axis;
 // a tile is `stripe` lines adjacent in x (a power of two, up to 32, that
 // fits the LDS tile) at one position of the remaining axis, so every global
-// row access covers `stripe` consecutive elements.
-// Input extent in_len (element i placed at wrap(i + in_off, L) of a zero
-// line when in_scatter), output extent out_len (element o taken from
-// wrap(o + out_off, L) when out_gather).
+// access of a wave covers rows of `stripe` consecutive elements.
+// The first stage reads its inputs from global memory (input extent in_len;
+// element i of the input sits at FFT index wrap(i + in_off, L) of an
+// otherwise zero line when in_scatter) and the last stage writes its
+// outputs there (output extent out_len; output o is FFT index
+// wrap(o + out_off, L) when out_gather); the stages between run in LDS.
 // ---------------------------------------------------------------------------
 struct AxisPass {
   const float2* in;
@@ -220,13 +252,14 @@ struct AxisPass {
   int in_len, out_len;
   int in_scatter, in_off;
   int out_gather, out_off;
-  int stripe, lg_stripe, x_tiles;
+  int stripe, x_tiles;
+  uint32_t m_stripe;
   float sign;
   FftRadices rad;
   const float2* tw;
 };
 
-__global__ void __launch_bounds__(kFftThreads) fft_axis_kernel(AxisPass p) {
+__global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.L;
@@ -240,30 +273,36 @@ __global__ void __launch_bounds__(kFftThreads) fft_axis_kernel(AxisPass p) {
   for (int i = threadIdx.x; i < p.L; i += kFftThreads) tw[i] = p.tw[i];
   const float2* in = p.in + coil * p.in_coil + ob * p.in_ob + x0;
   float2* out = p.out + coil * p.out_coil + ob * p.out_ob + x0;
-  if (p.in_scatter) {
-    for (int e = threadIdx.x; e < nl * p.LP; e += kFftThreads) buf[e] = make_float2(0.f, 0.f);
+  const int L = p.L, LP = p.LP;
+  // lines l >= nl of a partial tile are neither read nor written
+  auto get_global = [&](int l, int i) {
+    int idx = i;
+    if (p.in_scatter) {
+      idx = wrap(i - p.in_off, L);
+      if (idx >= p.in_len) return make_float2(0.f, 0.f);
+    }
+    return l < nl ? in[(long long)idx * p.in_step + l] : make_float2(0.f, 0.f);
+  };
+  auto put_global = [&](int l, int o, float2 v) {
+    if (p.out_gather) o = wrap(o - p.out_off, L);
+    if (l < nl && o < p.out_len) out[(long long)o * p.out_step + l] = v;
+  };
+  auto get_lds = [&](int l, int i) { return buf[l * LP + i]; };
+  auto put_lds = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
+  const int S = p.stripe;
+  const int n = p.rad.n;
+  __syncthreads();  // twiddles
+  if (n == 1) {
+    stage_any(S, p.m_stripe, p.rad.st[0], tw, p.sign, false, get_global, put_global);
+    return;
+  }
+  stage_any(S, p.m_stripe, p.rad.st[0], tw, p.sign, false, get_global, put_lds);
+  __syncthreads();
+  for (int i = 1; i < n - 1; i++) {
+    stage_any(S, p.m_stripe, p.rad.st[i], tw, p.sign, true, get_lds, put_lds);
     __syncthreads();
   }
-  const int n_in = p.in_len << p.lg_stripe;
-#pragma unroll 4
-  for (int e = threadIdx.x; e < n_in; e += kFftThreads) {
-    const int l = e & (p.stripe - 1), i = e >> p.lg_stripe;
-    if (l < nl) {
-      const int pos = p.in_scatter ? wrap(i + p.in_off, p.L) : i;
-      buf[l * p.LP + pos] = in[(long long)i * p.in_step + l];
-    }
-  }
-  __syncthreads();
-  fft_lines(buf, nl, p.LP, p.rad, tw, p.sign);
-  const int n_out = p.out_len << p.lg_stripe;
-#pragma unroll 4
-  for (int e = threadIdx.x; e < n_out; e += kFftThreads) {
-    const int l = e & (p.stripe - 1), o = e >> p.lg_stripe;
-    if (l < nl) {
-      const int src = p.out_gather ? wrap(o + p.out_off, p.L) : o;
-      out[(long long)o * p.out_step + l] = buf[l * p.LP + src];
-    }
-  }
+  stage_any(S, p.m_stripe, p.rad.st[n - 1], tw, p.sign, false, get_lds, put_global);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,7 +329,7 @@ struct LastPass {
   const float2* tw;
 };
 
-__global__ void __launch_bounds__(kFftThreads) fft_adjoint_last_kernel(LastPass p) {
+__global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPass p) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.G0;
@@ -301,13 +340,6 @@ __global__ void __launch_bounds__(kFftThreads) fft_adjoint_last_kernel(LastPass 
   const long long cin = (long long)p.G0 * p.lines;
   const long long pix0 = (long long)l0 * p.N0;
   const int npix = nq * p.N0;
-  float2 acc[kMaxPix];
-#pragma unroll
-  for (int k = 0; k < kMaxPix; k++) {
-    acc[k] = make_float2(0.f, 0.f);
-    const int e = threadIdx.x + k * kFftThreads;
-    if (p.sens && p.accumulate && e < npix) acc[k] = p.out[pix0 + e];
-  }
   const int per_coil = nq * p.G0;  // the Q lines of one coil are contiguous
   for (int c0 = 0; c0 < p.coils; c0 += p.Cg) {
     const int ng = min(p.Cg, p.coils - c0);
@@ -322,35 +354,35 @@ __global__ void __launch_bounds__(kFftThreads) fft_adjoint_last_kernel(LastPass 
       }
     }
     __syncthreads();
-    fft_lines(buf, ng * nq, p.LP, p.rad, tw, p.sign);
-#pragma unroll
-    for (int k = 0; k < kMaxPix; k++) {
-      const int e = threadIdx.x + k * kFftThreads;
-      if (e >= npix) continue;
+    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.rad, tw, p.sign);
+    // the coil sum of later coil groups (and of later calls when
+    // accumulating) adds to what this thread wrote for the same pixels
+    const bool add = c0 > 0 || p.accumulate;
+    for (int e = threadIdx.x; e < npix; e += kFftThreads) {
       const int q = qdiv(e, p.m_n0);
       const int x = e - q * p.N0;
       const long long pix = pix0 + e;
       const int m = wrap(x + p.crop0 + p.shift0, p.G0);
       const float d = p.apply_deapo ? p.deapo[pix] : 1.0f;
-      for (int c = 0; c < ng; c++) {
-        float2 v = buf[(c * nq + q) * p.LP + m];
-        v.x = (v.x * p.scale) * d;
-        v.y = (v.y * p.scale) * d;
-        if (p.sens) {
+      if (p.sens) {
+        float2 acc = add ? p.out[pix] : make_float2(0.f, 0.f);
+        for (int c = 0; c < ng; c++) {
+          float2 v = buf[(c * nq + q) * p.LP + m];
+          v.x = (v.x * p.scale) * d;
+          v.y = (v.y * p.scale) * d;
           const float2 s = p.sens[(long long)(c0 + c) * Ni + pix];
-          acc[k].x += v.x * s.x + v.y * s.y;
-          acc[k].y += v.y * s.x - v.x * s.y;
-        } else {
+          acc.x += v.x * s.x + v.y * s.y;
+          acc.y += v.y * s.x - v.x * s.y;
+        }
+        p.out[pix] = acc;
+      } else {
+        for (int c = 0; c < ng; c++) {
+          float2 v = buf[(c * nq + q) * p.LP + m];
+          v.x = (v.x * p.scale) * d;
+          v.y = (v.y * p.scale) * d;
           p.out[(long long)(c0 + c) * Ni + pix] = v;
         }
       }
-    }
-  }
-  if (p.sens) {
-#pragma unroll
-    for (int k = 0; k < kMaxPix; k++) {
-      const int e = threadIdx.x + k * kFftThreads;
-      if (e < npix) p.out[pix0 + e] = acc[k];
     }
   }
 }
@@ -371,13 +403,13 @@ struct FirstPass {
   int lines;
   int crop0, shift0;
   int coils, Cg, Q;
-  uint32_t m_g0, m_n0;
+  uint32_t m_g0, m_n0, m_pad;  // qdiv constants for G0, N0, G0 - N0
   float sign;
   FftRadices rad;
   const float2* tw;
 };
 
-__global__ void __launch_bounds__(kFftThreads) fft_forward_first_kernel(FirstPass p) {
+__global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(FirstPass p) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.G0;
@@ -392,12 +424,14 @@ __global__ void __launch_bounds__(kFftThreads) fft_forward_first_kernel(FirstPas
   for (int c0 = 0; c0 < p.coils; c0 += p.Cg) {
     const int ng = min(p.Cg, p.coils - c0);
     __syncthreads();  // the previous group's reads of buf are done
-    for (int e = threadIdx.x; e < ng * nq * p.LP; e += kFftThreads) buf[e] = make_float2(0.f, 0.f);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kMaxPix; k++) {
-      const int e = threadIdx.x + k * kFftThreads;
-      if (e >= npix) continue;
+    // zero padding: positions of "pixels" N0 .. G0-1 of every line
+    const int pad = p.G0 - p.N0;
+    for (int e = threadIdx.x; e < ng * nq * pad; e += kFftThreads) {
+      const int l = qdiv(e, p.m_pad);
+      const int x = p.N0 + e - l * pad;
+      buf[l * p.LP + wrap(x + p.crop0 - p.shift0, p.G0)] = make_float2(0.f, 0.f);
+    }
+    for (int e = threadIdx.x; e < npix; e += kFftThreads) {
       const int q = qdiv(e, p.m_n0);
       const int x = e - q * p.N0;
       const long long pix = pix0 + e;
@@ -411,7 +445,7 @@ __global__ void __launch_bounds__(kFftThreads) fft_forward_first_kernel(FirstPas
       }
     }
     __syncthreads();
-    fft_lines(buf, ng * nq, p.LP, p.rad, tw, p.sign);
+    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.rad, tw, p.sign);
     for (int c = 0; c < ng; c++) {
       float2* dst = p.out + (long long)(c0 + c) * cout + (long long)l0 * p.G0;
 #pragma unroll 4
@@ -446,14 +480,14 @@ bool make_radices(int L, FftRadices* rad) {
     s.Ns = Ns;
     s.per_line = L / s.R;
     s.tws = L / (Ns * s.R);
-    s.m_pl = qdiv_const(s.per_line);
     s.m_ns = qdiv_const(Ns);
     Ns *= s.R;
   }
   return true;
 }
 
-int pad_of(int L) { return L + 1; }
+// odd line pitch: the lanes of a wave, one line each, hit distinct banks
+int pad_of(int L) { return (L + 1) | 1; }
 
 size_t tile_bytes(int L, int lines) {
   return ((size_t)L + (size_t)lines * pad_of(L)) * sizeof(float2);
@@ -468,9 +502,10 @@ hipError_t allow_lds(K kernel, size_t lds) {
 
 // x-pass tile: Cg coils x Q lines with Cg * Q * LP <= kTileElems and
 // Q * N0 <= kMaxPix * kFftThreads (N0 <= G0 <= kMaxLen keeps Q >= 1 inside
-// both bounds).
+// both bounds); a multiple of 32 lines where that fits (one line per lane).
 void x_tiling(int G0, int N0, int coils, int* Cg, int* Q) {
-  const int per_tile = std::max(1, kTileElems / pad_of(G0));
+  int per_tile = std::max(1, kTileElems / pad_of(G0));
+  if (per_tile >= 32) per_tile -= per_tile % 32;
   *Cg = std::max(1, std::min(coils, per_tile));
   *Q = std::max(1, std::min(per_tile / *Cg, (kMaxPix * kFftThreads) / N0));
 }
@@ -478,9 +513,10 @@ void x_tiling(int G0, int N0, int coils, int* Cg, int* Q) {
 hipError_t launch_axis(AxisPass p, int coils, hipStream_t s) {
   p.LP = pad_of(p.L);
   if (!make_radices(p.L, &p.rad)) return hipErrorInvalidValue;
-  p.lg_stripe = 0;
-  while ((2 << p.lg_stripe) <= kMaxStripe && (2 << p.lg_stripe) * p.LP <= kTileElems) p.lg_stripe++;
-  p.stripe = 1 << p.lg_stripe;
+  int lg = 0;
+  while ((2 << lg) <= kMaxStripe && (2 << lg) * p.LP <= kTileElems) lg++;
+  p.stripe = 1 << lg;
+  p.m_stripe = qdiv_const(p.stripe);
   p.x_tiles = (p.nx + p.stripe - 1) / p.stripe;
   const size_t lds = tile_bytes(p.L, p.stripe);
   hipError_t e = allow_lds(fft_axis_kernel, lds);
@@ -636,6 +672,7 @@ hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil
   x_tiling(G0, f.N0, coils, &f.Cg, &f.Q);
   f.m_g0 = qdiv_const(G0);
   f.m_n0 = qdiv_const(f.N0);
+  f.m_pad = G0 > f.N0 ? qdiv_const(G0 - f.N0) : 0;
   f.sign = sign;
   if (!make_radices(G0, &f.rad)) return hipErrorInvalidValue;
   f.tw = tw + tw_off[0][0];
