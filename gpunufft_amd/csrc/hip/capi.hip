@@ -709,7 +709,7 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   if (rc == GNUFFT_OK) {
     std::vector<uint32_t> words;
     size_t base[6];
-    build_cover(g, &words, base, p->cover.max_len);
+    build_cover(g, &words, base, p->cover.max_len, &p->cover.max_cov);
     rc = copy_in((void**)&p->cover_d, words.data(), words.size() * sizeof(uint32_t), GNUFFT_HOST, s);
     if (rc == GNUFFT_OK) {
       for (int a = 0; a < 3; a++) {

@@ -191,11 +191,16 @@ struct Cover {
   const uint32_t* off[3];
   const uint32_t* ent[3];
   int max_len[3];  // longest list per axis
+  // most (candidate, periodic image) pairs covering one grid coordinate of
+  // one axis; the fold indexes them per coordinate when at most kCovMax
+  int max_cov;
 };
 
 // Host: build the per-axis lists of `g` as one array (offsets of axis a at
-// word base[a], entries at word base[3 + a]); max_len[a] = longest list.
-void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3]);
+// word base[a], entries at word base[3 + a]); max_len[a] = longest list,
+// max_cov as in Cover.
+void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3],
+                 int* max_cov);
 
 struct SpreadArgs {
   const float2* data;      // [coil][...] samples

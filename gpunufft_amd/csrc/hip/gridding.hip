@@ -462,8 +462,123 @@ __global__ void __launch_bounds__(256) reduce_chunks_kernel(const uint2* __restr
   reduce_body(scratch, kstride, t.x, t.y, (size_t)blockIdx.x * 256 + threadIdx.x);
 }
 
+// Fold of one home sector, driven by per-axis coverage tables: for each
+// grid coordinate of the sector along an axis, the (candidate, padded index)
+// pairs whose padded region covers it. A point visits exactly the sub-grids
+// covering it (one for most interior points, up to 2^d at sector corners
+// for P <= 2 sw), in a fixed (z, y, x) order: deterministic sums, and loads
+// only of words that contribute. Needs max_cov <= kCovMax (host-checked).
+constexpr int kCovMax = 4;
+constexpr int kFoldMaxH = 256;  // sector width bound (P <= 255)
+
 template <int DIM, int CB>
-__device__ inline void fold_body(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+__device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+  __shared__ int kofs[kFoldCands];                   // chunk of candidate (jz, jy, jx), -1 empty
+  __shared__ uint16_t cov[3][kFoldMaxH * kCovMax];   // candidate | padded index << 8
+  __shared__ uint8_t ncov[3][kFoldMaxH];
+  const int P = g.P;
+  const int PD = DIM == 3 ? P * P * P : P * P;
+  const int sw = g.sw;
+  const int coil0 = coil_block * CB;
+  const int sx = (int)(s % (uint32_t)g.S[0]);
+  const uint32_t r = s / (uint32_t)g.S[0];
+  const int sy = (int)(r % (uint32_t)g.S[1]);
+  const int sz = DIM == 3 ? (int)(r / (uint32_t)g.S[1]) : 0;
+  const int hs[3] = {sx, sy, sz};
+  const int h0[3] = {sx * sw, sy * sw, sz * sw};
+  const int hx = min(sw, g.G[0] - h0[0]);
+  const int hy = min(sw, g.G[1] - h0[1]);
+  const int hz = DIM == 3 ? min(sw, g.G[2] - h0[2]) : 1;
+  const int HP = hx * hy * hz;
+  const uint32_t cx0 = a.cover.off[0][sx], ncx = a.cover.off[0][sx + 1] - cx0;
+  const uint32_t cy0 = a.cover.off[1][sy], ncy = a.cover.off[1][sy + 1] - cy0;
+  const uint32_t cz0 = DIM == 3 ? a.cover.off[2][sz] : 0;
+  const uint32_t ncz = DIM == 3 ? a.cover.off[2][sz + 1] - cz0 : 1;
+  const int J = (int)(ncx * ncy * ncz);
+  const int tid = threadIdx.x, nt = (int)blockDim.x;
+  for (int j = tid; j < J; j += nt) {
+    const int jx = j % (int)ncx;
+    const int jy = (j / (int)ncx) % (int)ncy;
+    const int jz = j / (int)(ncx * ncy);
+    const int nx = (int)a.cover.ent[0][cx0 + jx];
+    const int ny = (int)a.cover.ent[1][cy0 + jy];
+    const int nz = DIM == 3 ? (int)a.cover.ent[2][cz0 + jz] : 0;
+    const int n = nx + g.S[0] * (ny + g.S[1] * nz);
+    const int k0 = (int)a.sector_chunks[n];
+    kofs[j] = (int)a.sector_chunks[n + 1] > k0 ? k0 : -1;
+  }
+  for (int e = tid; e < hx + hy + (DIM == 3 ? hz : 0); e += nt) {
+    const int ax = e < hx ? 0 : (e < hx + hy ? 1 : 2);
+    const int t = e - (ax == 0 ? 0 : (ax == 1 ? hx : hx + hy));
+    const int q = h0[ax] + t;
+    const uint32_t c0 = a.cover.off[ax][hs[ax]], nc = a.cover.off[ax][hs[ax] + 1] - c0;
+    int count = 0;
+    for (uint32_t j = 0; j < nc; j++) {
+      const int origin = (int)a.cover.ent[ax][c0 + j] * sw + sw / 2 - g.off;
+      for (int i = wrap(q - origin, g.G[ax]); i < P && count < kCovMax; i += g.G[ax])
+        cov[ax][t * kCovMax + count++] = (uint16_t)(j | ((uint32_t)i << 8));
+    }
+    ncov[ax][t] = (uint8_t)count;
+  }
+  __syncthreads();
+  const size_t kstride = (size_t)a.n_coils * PD;
+  const float2* scr = a.scratch + (size_t)coil0 * PD;
+  for (int base = 0; base < HP; base += nt * kFoldPts) {
+#pragma unroll
+    for (int p = 0; p < kFoldPts; p++) {
+      const int e = base + p * nt + tid;
+      if (e >= HP) continue;
+      const int tx = e % hx;
+      const int tyz = e / hx;
+      const int ty = tyz % hy;
+      const int tz = DIM == 3 ? tyz / hy : 0;
+      float ax_[CB], ay_[CB];
+#pragma unroll
+      for (int c = 0; c < CB; c++) {
+        ax_[c] = 0.0f;
+        ay_[c] = 0.0f;
+      }
+      const int nzc = DIM == 3 ? ncov[2][tz] : 1;
+      const int nyc = ncov[1][ty], nxc = ncov[0][tx];
+      for (int iz = 0; iz < nzc; iz++) {
+        const uint32_t vz = DIM == 3 ? cov[2][tz * kCovMax + iz] : 0u;
+        const int jz = (int)(vz & 255u), pz = (int)(vz >> 8);
+        for (int iy = 0; iy < nyc; iy++) {
+          const uint32_t vy = cov[1][ty * kCovMax + iy];
+          const int jyz = jz * (int)ncy + (int)(vy & 255u);
+          const int pyz = P * ((int)(vy >> 8) + P * pz);
+          for (int ix = 0; ix < nxc; ix++) {
+            const uint32_t vx = cov[0][tx * kCovMax + ix];
+            const int k = kofs[jyz * (int)ncx + (int)(vx & 255u)];
+            if (k < 0) continue;
+            const float2* src = scr + (size_t)k * kstride + pyz + (int)(vx >> 8);
+#pragma unroll
+            for (int c = 0; c < CB; c++) {
+              const float2 val = src[(size_t)min(c, a.n_coils - 1 - coil0) * PD];
+              ax_[c] += val.x;
+              ay_[c] += val.y;
+            }
+          }
+        }
+      }
+      int o0 = h0[0] + tx, o1 = h0[1] + ty, o2 = h0[2] + tz;
+      if (a.fold_shift) {
+        o0 = wrap(o0 - g.shift_inv[0], g.G[0]);
+        o1 = wrap(o1 - g.shift_inv[1], g.G[1]);
+        if (DIM == 3) o2 = wrap(o2 - g.shift_inv[2], g.G[2]);
+      }
+      const size_t gi = (size_t)o0 + (size_t)g.G[0] * ((size_t)o1 + (size_t)g.G[1] * o2);
+#pragma unroll
+      for (int c = 0; c < CB; c++)
+        if (coil0 + c < a.n_coils) a.grid[(size_t)(coil0 + c) * g.Ng + gi] = make_float2(ax_[c], ay_[c]);
+    }
+  }
+}
+
+// Fold driven by the item list of all (candidate, periodic image) pairs;
+// every point reads every item (masked). For geometries beyond kCovMax.
+template <int DIM, int CB>
+__device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
   // item per non-empty candidate sector: padded origin per axis and the
   // chunk holding the sector's (reduced) sub-grid; rep: packed recurrence
   // offsets (P > G only)
@@ -597,14 +712,24 @@ __device__ inline void fold_body(const Geom& g, const SpreadArgs& a, uint32_t s,
 }
 
 template <int DIM, int CB>
+__device__ inline void fold_body(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+  if (a.cover.max_cov <= kCovMax)
+    fold_body_cov<DIM, CB>(g, a, s, coil_block);
+  else
+    fold_body_generic<DIM, CB>(g, a, s, coil_block);
+}
+
+template <int DIM, int CB>
 __global__ void __launch_bounds__(kFoldThreads) fold_kernel(Geom g, SpreadArgs a) {
   fold_body<DIM, CB>(g, a, blockIdx.x, blockIdx.y);
 }
 
-void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3]) {
+void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3],
+                 int* max_cov) {
   std::vector<uint32_t>& w = *words;
   w.clear();
   std::vector<std::vector<uint32_t>> lists[3];
+  *max_cov = 1;
   for (int a = 0; a < 3; a++) {
     const int G = g.G[a], S = g.S[a];
     lists[a].assign((size_t)S, {});
@@ -622,6 +747,15 @@ void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], in
           meets = q >= lo && q < hi;
         }
         if (meets) lists[a][(size_t)s].push_back((uint32_t)n);
+      }
+      // (candidate, periodic image) pairs per coordinate of this sector
+      for (int q = lo; q < hi; q++) {
+        int count = 0;
+        for (uint32_t n : lists[a][(size_t)s]) {
+          const int first = (int)n * g.sw + g.sw / 2 - g.off;
+          for (int i = (((q - first) % G) + G) % G; i < g.P; i += G) count++;
+        }
+        *max_cov = std::max(*max_cov, count);
       }
     }
   }
