@@ -690,6 +690,10 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   g.dist_mult = (float)((double)(K - 1) * inv);
   g.Ng = (long long)g.G[0] * g.G[1] * g.G[2];
   g.Ni = (long long)g.N[0] * g.N[1] * g.N[2];
+  if (g.Ng >= (1ll << 31)) {
+    delete p;
+    return fail(GNUFFT_ERR_UNSUPPORTED, "plan_create: grid above 2^31 points per coil");
+  }
   g.fft_scale = 1.0f / (float)std::sqrt((float)g.Ng);
   g.M = 0;
   p->chunk = 1024;
@@ -1272,7 +1276,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     op.pre = PreArgs{nullptr, p->n_sens ? 0 : 1, p->n_sens ? p->sens_d : nullptr, p->deapo_d,
                      b->grid_d + grid_off[k], (int)C};
     op.R_spread = spread_round(p->g, kBatchCB, (int)p->round_target, half);
-    op.R_interp = interp_round(p->g, kBatchCB, (int)p->round_target, half);
+    op.R_interp = interp_round(p->g, kBatchCB, (int)p->round_target, half - 3072);
     if (op.R_spread <= 0 || op.R_interp <= 0) {
       rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: padded sector does not fit in LDS");
       break;

@@ -128,6 +128,12 @@ __device__ inline int qdiv(int n, uint32_t m) {
   return m ? (int)__umulhi((uint32_t)n, m) : n;
 }
 
+// p mod L for p in [-L, 2L)
+__device__ inline int wrap1(int p, int L) {
+  p = p < 0 ? p + L : p;
+  return p >= L ? p - L : p;
+}
+
 __device__ inline uint32_t qdiv_const_dev(int d) {
   return d == 1 ? 0u : (uint32_t)((0x100000000ull + (uint32_t)d - 1) / (uint32_t)d);
 }
@@ -274,18 +280,21 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
   const float2* in = p.in + coil * p.in_coil + ob * p.in_ob + x0;
   float2* out = p.out + coil * p.out_coil + ob * p.out_ob + x0;
   const int L = p.L, LP = p.LP;
+  // offsets within a coil's array fit 32 bits (grids below 2^31 points);
+  // in_off, out_off lie in (-L, L), so one conditional step wraps an index
+  const uint32_t in_step = (uint32_t)p.in_step, out_step = (uint32_t)p.out_step;
   // lines l >= nl of a partial tile are neither read nor written
   auto get_global = [&](int l, int i) {
     int idx = i;
     if (p.in_scatter) {
-      idx = wrap(i - p.in_off, L);
+      idx = wrap1(i - p.in_off, L);
       if (idx >= p.in_len) return make_float2(0.f, 0.f);
     }
-    return l < nl ? in[(long long)idx * p.in_step + l] : make_float2(0.f, 0.f);
+    return l < nl ? in[(uint32_t)idx * in_step + (uint32_t)l] : make_float2(0.f, 0.f);
   };
   auto put_global = [&](int l, int o, float2 v) {
-    if (p.out_gather) o = wrap(o - p.out_off, L);
-    if (l < nl && o < p.out_len) out[(long long)o * p.out_step + l] = v;
+    if (p.out_gather) o = wrap1(o - p.out_off, L);
+    if (l < nl && o < p.out_len) out[(uint32_t)o * out_step + (uint32_t)l] = v;
   };
   auto get_lds = [&](int l, int i) { return buf[l * LP + i]; };
   auto put_lds = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
@@ -362,7 +371,7 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPa
       const int q = qdiv(e, p.m_n0);
       const int x = e - q * p.N0;
       const long long pix = pix0 + e;
-      const int m = wrap(x + p.crop0 + p.shift0, p.G0);
+      const int m = wrap1(x + p.crop0 + p.shift0, p.G0);
       const float d = p.apply_deapo ? p.deapo[pix] : 1.0f;
       if (p.sens) {
         float2 acc = add ? p.out[pix] : make_float2(0.f, 0.f);
@@ -429,14 +438,14 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(First
     for (int e = threadIdx.x; e < ng * nq * pad; e += kFftThreads) {
       const int l = qdiv(e, p.m_pad);
       const int x = p.N0 + e - l * pad;
-      buf[l * p.LP + wrap(x + p.crop0 - p.shift0, p.G0)] = make_float2(0.f, 0.f);
+      buf[l * p.LP + wrap1(x + p.crop0 - p.shift0, p.G0)] = make_float2(0.f, 0.f);
     }
     for (int e = threadIdx.x; e < npix; e += kFftThreads) {
       const int q = qdiv(e, p.m_n0);
       const int x = e - q * p.N0;
       const long long pix = pix0 + e;
       const float d = p.deapo[pix];
-      const int pos = wrap(x + p.crop0 - p.shift0, p.G0);
+      const int pos = wrap1(x + p.crop0 - p.shift0, p.G0);
       const float2 shared_img = p.img_per_coil ? make_float2(0.f, 0.f) : p.img[pix];
       for (int c = 0; c < ng; c++) {
         float2 v = p.img_per_coil ? p.img[(long long)(c0 + c) * Ni + pix] : shared_img;

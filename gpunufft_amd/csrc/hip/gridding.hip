@@ -434,6 +434,7 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
 // Periodic wrap and the optional INVERSE fftshift of the store
 // (std_gpuNUFFT_kernels.cu:629-654) are folded into the indices.
 // ---------------------------------------------------------------------------
+constexpr size_t kInterpStaticLds = 4096;  // interp's static shared tables, rounded up
 constexpr int kFoldThreads = 256;
 constexpr int kFoldPts = 2;      // home points per thread per pass
 constexpr int kFoldCands = 256;  // candidate sectors per home sector (host-checked)
@@ -806,18 +807,39 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   int c3[3];
   load_center<DIM>(a.centers, ch.sector, c3);
 
+  // grid offset of padded index i along axis d (periodic wrap, FORWARD
+  // fftshift folded in): the padded point (x, y, z) reads grid element
+  // gix[0][x] + gix[1][y] + gix[2][z]
+  __shared__ int gix[3][256];
   for (int i = tid; i < g.K; i += NT) lut[i] = a.lut[i];
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
-  // Stage the padded sector (periodic wrap, FORWARD fftshift folded into the
-  // read index), element e = pt*CB + coil; zero the overrun slack.
-  for (int e = tid; e < L.sub_n; e += NT) {
-    const int pt = e / CB;
-    const int cc = e % CB;
-    const int cgf = coil0 + cc;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (pt < PD && cgf < a.n_coils)
-      v = a.grid[(size_t)cgf * g.Ng + padded_to_grid<DIM>(g, c3, pt, a.fold_shift ? 1 : 0)];
-    sub[e] = v;
+  for (int e = tid; e < DIM * P; e += NT) {
+    const int d = e >= 2 * P ? 2 : (e >= P ? 1 : 0);
+    const int i = e - d * P;
+    const int q = c3[d] - g.off + i + (a.fold_shift ? g.shift_fwd[d] : 0);
+    const int stride = d == 0 ? 1 : (d == 1 ? g.G[0] : g.G[0] * g.G[1]);
+    gix[d][i] = wrap(q, g.G[d]) * stride;
+  }
+  if (DIM == 2 && tid == 0) gix[2][0] = 0;
+  __syncthreads();
+  // Stage the padded sector, element e = pt*CB + coil (the coil of thread
+  // tid is tid % CB in every iteration); zero the overrun slack.
+  {
+    const int cc = tid % CB;
+    const bool coil_in = coil0 + cc < a.n_coils;
+    const float2* gbase = a.grid + (size_t)(coil_in ? coil0 + cc : 0) * g.Ng;
+    for (int e = tid; e < L.sub_n; e += NT) {
+      const int pt = e / CB;
+      float2 v = make_float2(0.0f, 0.0f);
+      if (pt < PD && coil_in) {
+        const int yz = (int)udiv((uint32_t)pt, g.mP);
+        const int x = pt - yz * P;
+        const int z = DIM == 3 ? (int)udiv((uint32_t)pt, g.mP2) : 0;
+        const int y = yz - z * P;
+        v = gbase[(uint32_t)(gix[0][x] + gix[1][y] + gix[2][z])];
+      }
+      sub[e] = v;
+    }
   }
 
   const int lane = tid & 63;
@@ -847,7 +869,6 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
       const int bx = (int)(packed & 255u);
       const int by = (int)((packed >> 8) & 255u);
       const int bz = (int)((packed >> 16) & 255u);
-      const int nx = wave_trip(packed, 0, g.kw);
       const int ny = wave_trip(packed, 1, g.kw);
       const int nz = DIM == 3 ? wave_trip(packed, 2, g.kw) : 1;
       float accx = 0.0f, accy = 0.0f;
@@ -862,13 +883,17 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
           const int iy = min(by + ty, Pmax);
           const float wzy = DIM == 3 ? wz * w[1 + NX + ty] : w[1 + NX + ty];
           const float2* row = sub + ((iz * P + iy) * P + bx) * CB + c;
+          // all NX words of the row in one batch of reads (taps past the
+          // sample's range have weight 0; the row window stays inside the
+          // sub-grid and its zeroed slack)
+          float2 v[NX];
+#pragma unroll
+          for (int tx = 0; tx < NX; tx++) v[tx] = row[tx * CB];
 #pragma unroll
           for (int tx = 0; tx < NX; tx++) {
-            if (tx >= nx) break;
             const float val = wzy * w[1 + tx];
-            const float2 v = row[tx * CB];
-            accx += v.x * val;
-            accy += v.y * val;
+            accx += v[tx].x * val;
+            accy += v[tx].y * val;
           }
         }
       }
@@ -995,8 +1020,9 @@ template <int DIM, int NX>
 static hipError_t interp_nx(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s) {
   // Prefer a footprint that lets two workgroups share a CU.
   int R = 0;
-  int cb = pick_cb<InterpLayout>(g, a.n_coils, 8, chunk, lds_limit() / 2, &R);
-  if (cb == 0) cb = pick_cb<InterpLayout>(g, a.n_coils, 8, chunk, lds_limit(), &R);
+  // room for the static tables (grid offsets, 3 KB) beside the dynamic LDS
+  int cb = pick_cb<InterpLayout>(g, a.n_coils, 8, chunk, lds_limit() / 2 - kInterpStaticLds, &R);
+  if (cb == 0) cb = pick_cb<InterpLayout>(g, a.n_coils, 8, chunk, lds_limit() - kInterpStaticLds, &R);
   switch (cb) {
     case 8: return interp_launch<DIM, NX, 8>(g, a, R, s);
     case 4: return interp_launch<DIM, NX, 4>(g, a, R, s);
