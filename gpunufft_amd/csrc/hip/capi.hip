@@ -204,7 +204,7 @@ struct gnufft_plan_s {
   // per-axis padded-sector cover lists of the grid (fold)
   uint32_t* cover_d = nullptr;
   Cover cover{};
-  // spread sub-grids, [chunk][coil][padded point], capacity in coils
+  // spread sub-grids, [chunk][padded point][coil], capacity in coils
   float2* scratch_d = nullptr;
   uint32_t scratch_coils = 0;
   // sensitivities
@@ -216,11 +216,11 @@ struct gnufft_plan_s {
   float2* img_tmp_d = nullptr;
   uint32_t img_tmp_coils = 0;
   std::map<uint32_t, FftPlans> ffts;
-  // fused FFT (GNUFFT_FFT_FUSED): twiddle tables [axis][sign][G_a]
+  // fused FFT (GNUFFT_FFT_FUSED): twiddles [axis][sign][G_a] and stage tables
   bool fused_ok = false;
   bool fused = false;
   float2* fft_tw_d = nullptr;
-  size_t fft_tw_off[3][2] = {};
+  size_t fft_tw_off[3][3] = {};
   StageTimer timer;
 };
 
@@ -708,7 +708,8 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
   if (rc == GNUFFT_OK && fused_fft_supported(g)) {
     const std::vector<float2> tw = fused_fft_twiddles(g, p->fft_tw_off);
     rc = copy_in((void**)&p->fft_tw_d, tw.data(), tw.size() * sizeof(float2), GNUFFT_HOST, s);
-    p->fused_ok = p->fused = rc == GNUFFT_OK;
+    p->fused_ok = rc == GNUFFT_OK;
+    p->fused = p->fused_ok && fused_fft_preferred(g);
   }
   if (rc == GNUFFT_OK) {
     std::vector<uint32_t> words;
@@ -1157,6 +1158,20 @@ struct gnufft_batch_s {
     FftPlans fft;
   };
   std::vector<Group> groups;
+  // plans on GNUFFT_FFT_FUSED: one descriptor per plan and one item list
+  // (plan, block) per pass; the other plans: rocFFT per grid size with the
+  // pre / post kernels
+  bool fused = false;            // any plan on the fused FFT
+  std::vector<char> op_fused;
+  std::vector<char> fft_desc_h;
+  void* fft_desc_d = nullptr;
+  uint2* fft_items_d = nullptr;
+  struct FftPass {
+    const uint2* items = nullptr;
+    uint32_t n = 0;
+    size_t lds = 0;
+  };
+  FftPass fft_adj[3], fft_fwd[3];
   hipEvent_t uploaded = nullptr;  // last descriptor upload (ops_h may be reused after it)
 };
 
@@ -1168,6 +1183,8 @@ void free_batch(gnufft_batch b) {
   dfree(b->items_d);
   dfree(b->grid_d);
   dfree(b->scratch_d);
+  dfree(b->fft_desc_d);
+  dfree(b->fft_items_d);
   for (auto& gr : b->groups) destroy_fft(gr.fft);
   delete b;
 }
@@ -1181,6 +1198,9 @@ int batch_wait_upload(gnufft_batch b) {
 int batch_upload_ops(gnufft_batch b, hipStream_t s) {
   HIP_TRY(hipMemcpyAsync(b->ops_d, b->ops_h.data(), b->ops_h.size() * sizeof(BatchOp),
                          hipMemcpyHostToDevice, s));
+  if (b->fused)
+    HIP_TRY(hipMemcpyAsync(b->fft_desc_d, b->fft_desc_h.data(), b->fft_desc_h.size(),
+                           hipMemcpyHostToDevice, s));
   if (!b->uploaded) HIP_TRY(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming));
   HIP_TRY(hipEventRecord(b->uploaded, s));
   return GNUFFT_OK;
@@ -1212,6 +1232,8 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
   if (!b) return fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: host allocation");
   b->dim = g0.dim;
   b->nx = nx;
+  b->fused = false;
+  for (uint32_t i = 0; i < n_plans; i++) b->fused = b->fused || plans[i]->fused;
   // batch order: grouped by grid size (one rocFFT plan per group)
   std::vector<uint32_t> order(n_plans);
   for (uint32_t i = 0; i < n_plans; i++) order[i] = i;
@@ -1230,14 +1252,21 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     b->plans.push_back(p);
     b->coils.push_back(C);
     b->caller_index.push_back(order[k]);
+    b->op_fused.push_back(p->fused ? 1 : 0);
     grid_off[k] = grid_total;
     grid_total += (size_t)C * p->g.Ng;
     scratch_off[k] = scratch_total;
-    scratch_total += (size_t)p->n_chunks * C * padded_points(p->g);
-    if (k == 0 || p->g.G[0] != plans[order[k - 1]]->g.G[0] || p->g.G[1] != plans[order[k - 1]]->g.G[1] ||
-        p->g.G[2] != plans[order[k - 1]]->g.G[2])
-      b->groups.push_back({k, 0, grid_off[k], FftPlans()});
-    b->groups.back().count++;
+    // spread sub-grids, then (fused FFT) the transform's intermediate array
+    scratch_total += std::max((size_t)p->n_chunks * C * padded_points(p->g),
+                              p->fused ? (size_t)C * fused_fft_tmp_elems(p->g) : (size_t)0);
+    // rocFFT groups: runs of consecutive non-fused plans of one grid size
+    if (!p->fused) {
+      const gnufft_plan q = k ? plans[order[k - 1]] : nullptr;
+      if (!q || q->fused || p->g.G[0] != q->g.G[0] || p->g.G[1] != q->g.G[1] ||
+          p->g.G[2] != q->g.G[2])
+        b->groups.push_back({k, 0, grid_off[k], FftPlans()});
+      b->groups.back().count++;
+    }
   }
   int rc = GNUFFT_OK;
   if (hipMalloc(&b->grid_d, std::max<size_t>(grid_total, 1) * sizeof(float2)) != hipSuccess ||
@@ -1291,6 +1320,16 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     b->fold_threads = std::max(b->fold_threads, ft);
     b->max_kstride = std::max<uint32_t>(b->max_kstride, C * (uint32_t)padded_points(p->g));
     b->ops_h.push_back(op);
+    if (b->fused) b->fft_desc_h.resize((k + 1) * fused_fft_desc_bytes());
+    if (p->fused) {
+      const size_t db = fused_fft_desc_bytes();
+      if (!fused_fft_desc_build(b->fft_desc_h.data() + k * db, p->g, b->grid_d + grid_off[k],
+                                b->scratch_d + scratch_off[k], (int)C, p->deapo_d,
+                                p->n_sens ? p->sens_d : nullptr, p->fft_tw_d, p->fft_tw_off)) {
+        rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: fused FFT descriptor");
+        break;
+      }
+    }
     // chunk list (host copy) for the gridding items and the reduce tasks
     std::vector<Chunk> ch(p->n_chunks);
     if (p->n_chunks &&
@@ -1314,8 +1353,10 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
         fo.push_back(make_uint4(k, sct, cb, 0));
         fo_cost.push_back(1u);
       }
-    for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ni; blk++) po.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
-    for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ng; blk++) pr.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
+    if (!p->fused) {  // pre / post run beside rocFFT; the fused FFT has them inside
+      for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ni; blk++) po.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
+      for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ng; blk++) pr.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
+    }
   }
   if (!rc) {
     std::vector<uint4> all;
@@ -1349,6 +1390,38 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     list(b->fold, fo, &fo_cost);
     list(b->post, po, nullptr);
     list(b->pre, pr, nullptr);
+    if (b->fused) {
+      // items (plan, block) of every pass of both directions, one array
+      const size_t db = fused_fft_desc_bytes();
+      const int np = fused_fft_pass_count(b->dim);
+      std::vector<uint2> fi;
+      std::vector<size_t> first;
+      for (int dir = 0; dir < 2; dir++)
+        for (int ps = 0; ps < np; ps++) {
+          gnufft_batch_s::FftPass& fp = dir == 0 ? b->fft_adj[ps] : b->fft_fwd[ps];
+          first.push_back(fi.size());
+          for (uint32_t k = 0; k < n_plans; k++) {
+            if (!b->op_fused[k]) continue;
+            const void* d = b->fft_desc_h.data() + k * db;
+            const uint32_t nb = fused_fft_pass_blocks(d, dir == 0, ps);
+            for (uint32_t blk = 0; blk < nb; blk++) fi.push_back(make_uint2(k, blk));
+            fp.lds = std::max(fp.lds, fused_fft_pass_lds(d, dir == 0, ps));
+          }
+          fp.n = (uint32_t)(fi.size() - first.back());
+        }
+      if (hipMalloc(&b->fft_desc_d, std::max<size_t>(b->fft_desc_h.size(), 1)) != hipSuccess ||
+          hipMalloc(&b->fft_items_d, std::max<size_t>(fi.size(), 1) * sizeof(uint2)) != hipSuccess) {
+        rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: FFT descriptor allocation");
+      } else if (hipMemcpyAsync(b->fft_items_d, fi.data(), fi.size() * sizeof(uint2),
+                                hipMemcpyHostToDevice, s) != hipSuccess) {
+        rc = fail(GNUFFT_ERR_DEVICE, "batch_create: FFT item upload");
+      } else {
+        size_t j = 0;
+        for (int dir = 0; dir < 2; dir++)
+          for (int ps = 0; ps < np; ps++)
+            (dir == 0 ? b->fft_adj[ps] : b->fft_fwd[ps]).items = b->fft_items_d + first[j++];
+      }
+    }
     for (auto& gr : b->groups) {
       uint32_t coils = 0;
       for (uint32_t k = gr.first; k < gr.first + gr.count; k++) coils += b->coils[k];
@@ -1382,12 +1455,20 @@ int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* 
     if (!kdata[ci] || !out[ci]) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_adjoint: null buffer");
     b->ops_h[k].s.data = (const float2*)kdata[ci];
     b->ops_h[k].post.out = (float2*)out[ci];
+    if (b->op_fused[k])
+      fused_fft_desc_set_io(b->fft_desc_h.data() + k * fused_fft_desc_bytes(), (float2*)out[ci],
+                            nullptr);
   }
   int rc = batch_upload_ops(b, s);
   if (rc) return rc;
   HIP_TRY(launch_spread_batch(b->dim, b->nx, b->ops_d, b->spread, b->spread_lds, s));
   HIP_TRY(launch_reduce_batch(b->ops_d, b->reduce, b->max_kstride, s));
   HIP_TRY(launch_fold_batch(b->dim, b->ops_d, b->fold, b->fold_threads, s));
+  if (b->fused) {
+    for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++)
+      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, b->fft_desc_d, b->fft_adj[ps].items,
+                                     b->fft_adj[ps].n, b->fft_adj[ps].lds, s));
+  }
   for (auto& gr : b->groups) {
     rc = exec_fft(gr.fft, true, b->grid_d + gr.offset, s);
     if (rc) return rc;
@@ -1406,6 +1487,9 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
     if (!img[ci] || !out[ci]) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_forward: null buffer");
     b->ops_h[k].pre.img = (const float2*)img[ci];
     b->ops_h[k].i.out = (float2*)out[ci];
+    if (b->op_fused[k])
+      fused_fft_desc_set_io(b->fft_desc_h.data() + k * fused_fft_desc_bytes(), nullptr,
+                            (const float2*)img[ci]);
   }
   int rc = batch_upload_ops(b, s);
   if (rc) return rc;
@@ -1413,6 +1497,11 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
   for (auto& gr : b->groups) {
     rc = exec_fft(gr.fft, false, b->grid_d + gr.offset, s);
     if (rc) return rc;
+  }
+  if (b->fused) {
+    for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++)
+      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, b->fft_desc_d, b->fft_fwd[ps].items,
+                                     b->fft_fwd[ps].n, b->fft_fwd[ps].lds, s));
   }
   HIP_TRY(launch_interp_batch(b->dim, b->nx, b->ops_d, b->interp, b->interp_lds, s));
   return GNUFFT_OK;

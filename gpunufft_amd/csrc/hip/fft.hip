@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "gnufft_internal.hpp"
@@ -42,26 +43,26 @@ constexpr int kMaxLen = kMaxPix * kFftThreads;  // longest transform (2048)
 // ---------------------------------------------------------------------------
 // Small DFTs with kernel exp(sign * 2 pi i / R)
 // ---------------------------------------------------------------------------
-__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ inline float2 cmul(float2 a, float2 b) {
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // sign * i * a
-__device__ inline float2 mul_si(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
+__device__ __forceinline__ float2 mul_si(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
 
 template <int R>
-__device__ inline void dft(float2* v, float sign);
+__device__ __forceinline__ void dft(float2* v, float sign);
 
 template <>
-__device__ inline void dft<2>(float2* v, float) {
+__device__ __forceinline__ void dft<2>(float2* v, float) {
   const float2 a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
 
 template <>
-__device__ inline void dft<4>(float2* v, float sign) {
+__device__ __forceinline__ void dft<4>(float2* v, float sign) {
   const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
   const float2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sign);
   v[0] = cadd(t0, t2);
@@ -71,7 +72,7 @@ __device__ inline void dft<4>(float2* v, float sign) {
 }
 
 template <>
-__device__ inline void dft<8>(float2* v, float sign) {
+__device__ __forceinline__ void dft<8>(float2* v, float sign) {
   // radix-4 DFTs of the even and odd inputs, combined with w8^k
   float2 e[4] = {v[0], v[2], v[4], v[6]};
   float2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -93,7 +94,7 @@ __device__ inline void dft<8>(float2* v, float sign) {
 }
 
 template <>
-__device__ inline void dft<3>(float2* v, float sign) {
+__device__ __forceinline__ void dft<3>(float2* v, float sign) {
   const float s = 0.86602540378443865f * sign;  // sin(2 pi / 3)
   const float2 a = v[0];
   const float2 t = cadd(v[1], v[2]);
@@ -105,7 +106,7 @@ __device__ inline void dft<3>(float2* v, float sign) {
 }
 
 template <>
-__device__ inline void dft<5>(float2* v, float sign) {
+__device__ __forceinline__ void dft<5>(float2* v, float sign) {
   const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;  // cos 72, cos 144
   const float s1 = 0.95105651629515357f * sign, s2 = 0.58778525229247313f * sign;
   const float2 a = v[0];
@@ -124,17 +125,17 @@ __device__ inline void dft<5>(float2* v, float sign) {
 
 // n / d for n * d < 2^32 via a multiply-high with m = ceil(2^32 / d); d == 1
 // has no 32-bit m and is passed as m == 0.
-__device__ inline int qdiv(int n, uint32_t m) {
+__device__ __forceinline__ int qdiv(int n, uint32_t m) {
   return m ? (int)__umulhi((uint32_t)n, m) : n;
 }
 
 // p mod L for p in [-L, 2L)
-__device__ inline int wrap1(int p, int L) {
+__device__ __forceinline__ int wrap1(int p, int L) {
   p = p < 0 ? p + L : p;
   return p >= L ? p - L : p;
 }
 
-__device__ inline uint32_t qdiv_const_dev(int d) {
+__device__ __forceinline__ uint32_t qdiv_const_dev(int d) {
   return d == 1 ? 0u : (uint32_t)((0x100000000ull + (uint32_t)d - 1) / (uint32_t)d);
 }
 
@@ -150,6 +151,7 @@ __device__ inline uint32_t qdiv_const_dev(int d) {
 struct FftStage {
   int R, Ns, per_line, tws;
   uint32_t m_ns;  // qdiv constant for Ns
+  uint32_t pad;   // 24 bytes: records pack into float2 slots of the table
 };
 
 struct FftRadices {
@@ -165,7 +167,7 @@ struct Bfly {
 // Twiddle, radix-R DFT and store of butterfly j (values v) at
 // (j / Ns) Ns R + k + s Ns of line l, through `put(l, index, value)`.
 template <int R, typename Put>
-__device__ inline void finish_butterfly(float2* v, int j, int l, const FftStage& st,
+__device__ __forceinline__ void finish_butterfly(float2* v, int j, int l, const FftStage& st,
                                         const float2* tw, float sign, Put put) {
   const int jq = qdiv(j, st.m_ns);
   const int k = j - jq * st.Ns;
@@ -184,7 +186,7 @@ __device__ inline void finish_butterfly(float2* v, int j, int l, const FftStage&
 // put(l, index, value); a barrier separates all fetches from all stores
 // when `sync` (in-place LDS stages).
 template <int R, typename Get, typename Put>
-__device__ inline void stage(int lines, uint32_t m_lines, const FftStage& st, const float2* tw,
+__device__ __forceinline__ void stage(int lines, uint32_t m_lines, const FftStage& st, const float2* tw,
                              float sign, bool sync, Get get, Put put) {
   constexpr int KB = Bfly<R>::KB;
   const int B = lines * st.per_line;
@@ -212,8 +214,11 @@ __device__ inline void stage(int lines, uint32_t m_lines, const FftStage& st, co
 }
 
 template <typename Get, typename Put>
-__device__ inline void stage_any(int lines, uint32_t m_lines, const FftStage& st,
+__device__ __forceinline__ void stage_any(int lines, uint32_t m_lines, const FftStage& table_entry,
                                  const float2* tw, float sign, bool sync, Get get, Put put) {
+  // a register copy: the table lives in global memory, and stores of the
+  // stage to global memory would otherwise force reloads of its fields
+  const FftStage st = table_entry;
   switch (st.R) {
     case 8: stage<8>(lines, m_lines, st, tw, sign, sync, get, put); break;
     case 4: stage<4>(lines, m_lines, st, tw, sign, sync, get, put); break;
@@ -225,12 +230,12 @@ __device__ inline void stage_any(int lines, uint32_t m_lines, const FftStage& st
 
 // Transform `lines` lines held in LDS (stride LP) in place. Callers
 // synchronise after filling the lines; every stage ends with a barrier.
-__device__ inline void fft_lines(float2* buf, int lines, uint32_t m_lines, int LP,
-                                 const FftRadices& rad, const float2* tw, float sign) {
+__device__ __forceinline__ void fft_lines(float2* buf, int lines, uint32_t m_lines, int LP,
+                                 const FftStage* st, int n_st, const float2* tw, float sign) {
   auto get = [&](int l, int i) { return buf[l * LP + i]; };
   auto put = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
-  for (int i = 0; i < rad.n; i++) {
-    stage_any(lines, m_lines, rad.st[i], tw, sign, true, get, put);
+  for (int i = 0; i < n_st; i++) {
+    stage_any(lines, m_lines, st[i], tw, sign, true, get, put);
     __syncthreads();
   }
 }
@@ -258,20 +263,21 @@ struct AxisPass {
   int in_len, out_len;
   int in_scatter, in_off;
   int out_gather, out_off;
-  int stripe, x_tiles;
+  int stripe, x_tiles, coils;
   uint32_t m_stripe;
   float sign;
-  FftRadices rad;
+  const FftStage* st;  // stage table in device memory (beside the twiddles)
+  int n_st;
   const float2* tw;
 };
 
-__global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
+__device__ __forceinline__ void fft_axis_body(const AxisPass& p, int block) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.L;
-  // blockIdx.x = (coil * nob + ob) * x_tiles + xt
-  const int xt = blockIdx.x % p.x_tiles;
-  const int r = blockIdx.x / p.x_tiles;
+  // block = (coil * nob + ob) * x_tiles + xt
+  const int xt = block % p.x_tiles;
+  const int r = block / p.x_tiles;
   const int ob = r % p.nob;
   const int coil = r / p.nob;
   const int x0 = xt * p.stripe;
@@ -299,19 +305,23 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
   auto get_lds = [&](int l, int i) { return buf[l * LP + i]; };
   auto put_lds = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
   const int S = p.stripe;
-  const int n = p.rad.n;
+  const int n = p.n_st;
   __syncthreads();  // twiddles
   if (n == 1) {
-    stage_any(S, p.m_stripe, p.rad.st[0], tw, p.sign, false, get_global, put_global);
+    stage_any(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_global);
     return;
   }
-  stage_any(S, p.m_stripe, p.rad.st[0], tw, p.sign, false, get_global, put_lds);
+  stage_any(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_lds);
   __syncthreads();
   for (int i = 1; i < n - 1; i++) {
-    stage_any(S, p.m_stripe, p.rad.st[i], tw, p.sign, true, get_lds, put_lds);
+    stage_any(S, p.m_stripe, p.st[i], tw, p.sign, true, get_lds, put_lds);
     __syncthreads();
   }
-  stage_any(S, p.m_stripe, p.rad.st[n - 1], tw, p.sign, false, get_lds, put_global);
+  stage_any(S, p.m_stripe, p.st[n - 1], tw, p.sign, false, get_lds, put_global);
+}
+
+__global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
+  fft_axis_body(p, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -334,15 +344,16 @@ struct LastPass {
   float scale;
   int apply_deapo, accumulate;
   float sign;
-  FftRadices rad;
+  const FftStage* st;  // stage table in device memory (beside the twiddles)
+  int n_st;
   const float2* tw;
 };
 
-__global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPass p) {
+__device__ __forceinline__ void fft_adjoint_last_body(const LastPass& p, int block) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.G0;
-  const int l0 = blockIdx.x * p.Q;
+  const int l0 = block * p.Q;
   const int nq = min(p.Q, p.lines - l0);
   for (int i = threadIdx.x; i < p.G0; i += kFftThreads) tw[i] = p.tw[i];
   const long long Ni = (long long)p.N0 * p.lines;
@@ -363,7 +374,7 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPa
       }
     }
     __syncthreads();
-    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.rad, tw, p.sign);
+    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.st, p.n_st, tw, p.sign);
     // the coil sum of later coil groups (and of later calls when
     // accumulating) adds to what this thread wrote for the same pixels
     const bool add = c0 > 0 || p.accumulate;
@@ -414,15 +425,16 @@ struct FirstPass {
   int coils, Cg, Q;
   uint32_t m_g0, m_n0, m_pad;  // qdiv constants for G0, N0, G0 - N0
   float sign;
-  FftRadices rad;
+  const FftStage* st;  // stage table in device memory (beside the twiddles)
+  int n_st;
   const float2* tw;
 };
 
-__global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(FirstPass p) {
+__device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int block) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.G0;
-  const int l0 = blockIdx.x * p.Q;
+  const int l0 = block * p.Q;
   const int nq = min(p.Q, p.lines - l0);
   for (int i = threadIdx.x; i < p.G0; i += kFftThreads) tw[i] = p.tw[i];
   const long long Ni = (long long)p.N0 * p.lines;
@@ -454,7 +466,7 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(First
       }
     }
     __syncthreads();
-    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.rad, tw, p.sign);
+    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.st, p.n_st, tw, p.sign);
     for (int c = 0; c < ng; c++) {
       float2* dst = p.out + (long long)(c0 + c) * cout + (long long)l0 * p.G0;
 #pragma unroll 4
@@ -465,6 +477,52 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(First
       }
     }
   }
+}
+
+__global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPass p) {
+  fft_adjoint_last_body(p, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(FirstPass p) {
+  fft_forward_first_body(p, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batches: the passes of many operators, one launch per pass. An item
+// (operator, block) names a workgroup's tile; FftOpDesc holds the
+// operator's pass descriptors in device memory.
+// ---------------------------------------------------------------------------
+struct FftOpDesc {
+  AxisPass adj_ax[2];  // adjoint z (3-D), y
+  LastPass adj_x;
+  FirstPass fwd_x;
+  AxisPass fwd_ax[2];  // forward y, z (3-D)
+  int n_ax;            // d - 1
+};
+
+__global__ void __launch_bounds__(kFftThreads, 4) fft_axis_batch_kernel(const FftOpDesc* __restrict__ d,
+                                                                        const uint2* __restrict__ items,
+                                                                        int adjoint, int which) {
+  const uint2 it = items[blockIdx.x];
+  const FftOpDesc& o = d[it.x];
+  // register copies: the descriptors live in global memory, which the
+  // passes also store to
+  const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
+  fft_axis_body(p, (int)it.y);
+}
+
+__global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_batch_kernel(
+    const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
+  const uint2 it = items[blockIdx.x];
+  const LastPass p = d[it.x].adj_x;
+  fft_adjoint_last_body(p, (int)it.y);
+}
+
+__global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_batch_kernel(
+    const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
+  const uint2 it = items[blockIdx.x];
+  const FirstPass p = d[it.x].fwd_x;
+  fft_forward_first_body(p, (int)it.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,72 +577,56 @@ void x_tiling(int G0, int N0, int coils, int* Cg, int* Q) {
   *Q = std::max(1, std::min(per_tile / *Cg, (kMaxPix * kFftThreads) / N0));
 }
 
-hipError_t launch_axis(AxisPass p, int coils, hipStream_t s) {
-  p.LP = pad_of(p.L);
-  if (!make_radices(p.L, &p.rad)) return hipErrorInvalidValue;
-  int lg = 0;
-  while ((2 << lg) <= kMaxStripe && (2 << lg) * p.LP <= kTileElems) lg++;
-  p.stripe = 1 << lg;
-  p.m_stripe = qdiv_const(p.stripe);
-  p.x_tiles = (p.nx + p.stripe - 1) / p.stripe;
-  const size_t lds = tile_bytes(p.L, p.stripe);
-  hipError_t e = allow_lds(fft_axis_kernel, lds);
-  if (e != hipSuccess) return e;
-  const unsigned blocks = (unsigned)((long long)coils * p.nob * p.x_tiles);
-  hipLaunchKernelGGL(fft_axis_kernel, dim3(blocks), dim3(kFftThreads), lds, s, p);
-  return hipGetLastError();
+// The stage table of axis a inside the device table buffer.
+const FftStage* stage_table(const float2* tw, const size_t tw_off[3][3], int a) {
+  return reinterpret_cast<const FftStage*>(tw + tw_off[a][2]);
 }
 
-}  // namespace
-
-bool fused_fft_supported(const Geom& g) {
+// Tiling of a y / z pass (the caller fills the geometry fields).
+bool finish_axis(AxisPass* p, int coils, const FftStage* st) {
+  p->coils = coils;
+  p->LP = pad_of(p->L);
   FftRadices r;
-  for (int a = 0; a < g.dim; a++) {
-    // one padded line fits a tile; the x tiling needs N0 <= G0 <= kMaxLen
-    if (g.G[a] > kMaxLen || !make_radices(g.G[a], &r)) return false;
-    if (g.N[a] > g.G[a]) return false;
-  }
+  if (!make_radices(p->L, &r)) return false;
+  p->st = st;
+  p->n_st = r.n;
+  int lg = 0;
+  while ((2 << lg) <= kMaxStripe && (2 << lg) * p->LP <= kTileElems) lg++;
+  p->stripe = 1 << lg;
+  p->m_stripe = qdiv_const(p->stripe);
+  p->x_tiles = (p->nx + p->stripe - 1) / p->stripe;
   return true;
 }
 
-size_t fused_fft_tmp_elems(const Geom& g) {
-  // adjoint z-pass / forward y-pass output (3-D), x-pass output (2-D)
-  return g.dim == 3 ? (size_t)g.G[0] * g.G[1] * g.N[2] : (size_t)g.G[0] * g.N[1];
+uint32_t axis_blocks(const AxisPass& p) { return (uint32_t)p.coils * p.nob * p.x_tiles; }
+size_t axis_lds(const AxisPass& p) { return tile_bytes(p.L, p.stripe); }
+uint32_t last_blocks(const LastPass& p) { return (uint32_t)((p.lines + p.Q - 1) / p.Q); }
+size_t last_lds(const LastPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
+uint32_t first_blocks(const FirstPass& p) { return (uint32_t)((p.lines + p.Q - 1) / p.Q); }
+size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
+
+hipError_t launch_axis(const AxisPass& p, hipStream_t s) {
+  const size_t lds = axis_lds(p);
+  hipError_t e = allow_lds(fft_axis_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fft_axis_kernel, dim3(axis_blocks(p)), dim3(kFftThreads), lds, s, p);
+  return hipGetLastError();
 }
 
-// exp(sign 2 pi i k / G_a) for k < G_a, computed in double, laid out
-// [axis][sign -1, +1][G_a]; offsets[a][0 | 1] index the table.
-std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][2]) {
-  std::vector<float2> t;
-  for (int a = 0; a < 3; a++) offsets[a][0] = offsets[a][1] = 0;
-  for (int a = 0; a < g.dim; a++)
-    for (int si = 0; si < 2; si++) {
-      offsets[a][si] = t.size();
-      const double sign = si == 0 ? -1.0 : 1.0;
-      const int L = g.G[a];
-      for (int k = 0; k < L; k++) {
-        const double ang = sign * 2.0 * M_PI * (double)k / (double)L;
-        t.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
-      }
-    }
-  return t;
-}
-
-// Adjoint: grid [coils][G2][G1][G0] (the INVERSE-shifted spectrum) -> image.
-// tmp holds coils * fused_fft_tmp_elems(g) elements; the grid is
-// overwritten. stop == 1 (FFT): per-coil images scaled by 1/sqrt(Ng) only.
-// accumulate: add the coil sum to `out` (later coil batches).
-hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,
-                             const float* deapo, const float2* sens, float2* out, int stop,
-                             bool accumulate, const float2* tw, const size_t tw_off[3][2],
-                             hipStream_t s) {
+// Pass descriptors of the adjoint: grid [coils][G2][G1][G0] (the
+// INVERSE-shifted spectrum) -> image; ax[0 .. d-2] then the x pass.
+bool build_adjoint(const Geom& g, float2* grid, float2* tmp, int coils, const float* deapo,
+                   const float2* sens, float2* out, int stop, bool accumulate, const float2* tw,
+                   const size_t tw_off[3][3], AxisPass ax[2], LastPass* last) {
   const float sign = 1.0f;  // unnormalised inverse transform
   const int G0 = g.G[0], G1 = g.G[1];
   const int N1 = g.N[1], N2 = g.dim == 3 ? g.N[2] : 1;
   const float2* cur = grid;
+  int n = 0;
   if (g.dim == 3) {
     // z: [c][G2][G1][G0] -> tmp [c][N2][G1][G0]
-    AxisPass z{};
+    AxisPass& z = ax[n++];
+    z = AxisPass{};
     z.in = grid;
     z.out = tmp;
     z.in_coil = (long long)G0 * G1 * g.G[2];
@@ -600,34 +642,32 @@ hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils
     z.out_off = g.crop[2] + g.shift_inv[2];
     z.sign = sign;
     z.tw = tw + tw_off[2][1];
-    hipError_t e = launch_axis(z, coils, s);
-    if (e != hipSuccess) return e;
+    if (!finish_axis(&z, coils, stage_table(tw, tw_off, 2))) return false;
     cur = tmp;
   }
   // y: [c][N2][G1][G0] -> [c][N2][N1][G0] (3-D: tmp -> grid, 2-D: grid -> tmp)
   float2* ydst = g.dim == 3 ? grid : tmp;
-  {
-    AxisPass y{};
-    y.in = cur;
-    y.out = ydst;
-    y.in_coil = (long long)G0 * G1 * N2;
-    y.out_coil = (long long)G0 * N1 * N2;
-    y.in_step = y.out_step = G0;
-    y.in_ob = (long long)G0 * G1;
-    y.out_ob = (long long)G0 * N1;
-    y.nx = G0;
-    y.nob = N2;
-    y.L = G1;
-    y.in_len = G1;
-    y.out_len = N1;
-    y.out_gather = 1;
-    y.out_off = g.crop[1] + g.shift_inv[1];
-    y.sign = sign;
-    y.tw = tw + tw_off[1][1];
-    hipError_t e = launch_axis(y, coils, s);
-    if (e != hipSuccess) return e;
-  }
-  LastPass p{};
+  AxisPass& y = ax[n++];
+  y = AxisPass{};
+  y.in = cur;
+  y.out = ydst;
+  y.in_coil = (long long)G0 * G1 * N2;
+  y.out_coil = (long long)G0 * N1 * N2;
+  y.in_step = y.out_step = G0;
+  y.in_ob = (long long)G0 * G1;
+  y.out_ob = (long long)G0 * N1;
+  y.nx = G0;
+  y.nob = N2;
+  y.L = G1;
+  y.in_len = G1;
+  y.out_len = N1;
+  y.out_gather = 1;
+  y.out_off = g.crop[1] + g.shift_inv[1];
+  y.sign = sign;
+  y.tw = tw + tw_off[1][1];
+  if (!finish_axis(&y, coils, stage_table(tw, tw_off, 1))) return false;
+  LastPass& p = *last;
+  p = LastPass{};
   p.in = ydst;
   p.out = out;
   p.deapo = deapo;
@@ -646,27 +686,24 @@ hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils
   p.apply_deapo = stop == 1 ? 0 : 1;
   p.accumulate = accumulate ? 1 : 0;
   p.sign = sign;
-  if (!make_radices(G0, &p.rad)) return hipErrorInvalidValue;
   p.tw = tw + tw_off[0][1];
-  const size_t lds = tile_bytes(G0, p.Q * p.Cg);
-  hipError_t e = allow_lds(fft_adjoint_last_kernel, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fft_adjoint_last_kernel, dim3((unsigned)((p.lines + p.Q - 1) / p.Q)),
-                     dim3(kFftThreads), lds, s, p);
-  return hipGetLastError();
+  FftRadices r;
+  if (!make_radices(G0, &r)) return false;
+  p.st = stage_table(tw, tw_off, 0);
+  p.n_st = r.n;
+  return true;
 }
 
-// Forward: image -> grid [coils][G2][G1][G0], the spectrum of the
-// zero-padded, INVERSE-shifted, deapodized (and coil-weighted) image. tmp
-// holds coils * fused_fft_tmp_elems(g) elements.
-hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil,
-                             const float2* sens, const float* deapo, float2* grid, float2* tmp,
-                             int coils, const float2* tw, const size_t tw_off[3][2],
-                             hipStream_t s) {
+// Pass descriptors of the forward: image -> grid [coils][G2][G1][G0]; the x
+// pass, then ax[0 .. d-2] (y, z).
+bool build_forward(const Geom& g, const float2* img, bool img_per_coil, const float2* sens,
+                   const float* deapo, float2* grid, float2* tmp, int coils, const float2* tw,
+                   const size_t tw_off[3][3], FirstPass* first, AxisPass ax[2]) {
   const float sign = -1.0f;
   const int G0 = g.G[0], G1 = g.G[1];
   const int N1 = g.N[1], N2 = g.dim == 3 ? g.N[2] : 1;
-  FirstPass f{};
+  FirstPass& f = *first;
+  f = FirstPass{};
   f.img = img;
   f.deapo = deapo;
   f.sens = sens;
@@ -683,20 +720,17 @@ hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil
   f.m_n0 = qdiv_const(f.N0);
   f.m_pad = G0 > f.N0 ? qdiv_const(G0 - f.N0) : 0;
   f.sign = sign;
-  if (!make_radices(G0, &f.rad)) return hipErrorInvalidValue;
   f.tw = tw + tw_off[0][0];
+  FftRadices r;
+  if (!make_radices(G0, &r)) return false;
+  f.st = stage_table(tw, tw_off, 0);
+  f.n_st = r.n;
   // 3-D: x -> grid (as [c][N2][N1][G0]), y -> tmp, z -> grid
   // 2-D: x -> tmp, y -> grid
   f.out = g.dim == 3 ? grid : tmp;
-  const size_t lds = tile_bytes(G0, f.Q * f.Cg);
-  hipError_t e = allow_lds(fft_forward_first_kernel, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fft_forward_first_kernel, dim3((unsigned)((f.lines + f.Q - 1) / f.Q)),
-                     dim3(kFftThreads), lds, s, f);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
   // y: [c][N2][N1][G0] -> [c][N2][G1][G0]
-  AxisPass y{};
+  AxisPass& y = ax[0];
+  y = AxisPass{};
   y.in = f.out;
   y.out = g.dim == 3 ? tmp : grid;
   y.in_coil = (long long)G0 * N1 * N2;
@@ -713,10 +747,11 @@ hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil
   y.in_off = g.crop[1] - g.shift_inv[1];
   y.sign = sign;
   y.tw = tw + tw_off[1][0];
-  e = launch_axis(y, coils, s);
-  if (e != hipSuccess || g.dim == 2) return e;
+  if (!finish_axis(&y, coils, stage_table(tw, tw_off, 1))) return false;
+  if (g.dim == 2) return true;
   // z: [c][N2][G1][G0] -> [c][G2][G1][G0]
-  AxisPass z{};
+  AxisPass& z = ax[1];
+  z = AxisPass{};
   z.in = tmp;
   z.out = grid;
   z.in_coil = (long long)G0 * G1 * N2;
@@ -732,7 +767,176 @@ hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil
   z.in_off = g.crop[2] - g.shift_inv[2];
   z.sign = sign;
   z.tw = tw + tw_off[2][0];
-  return launch_axis(z, coils, s);
+  return finish_axis(&z, coils, stage_table(tw, tw_off, 2));
+}
+
+}  // namespace
+
+bool fused_fft_supported(const Geom& g) {
+  FftRadices r;
+  for (int a = 0; a < g.dim; a++) {
+    // one padded line fits a tile; the x tiling needs N0 <= G0 <= kMaxLen
+    if (g.G[a] > kMaxLen || !make_radices(g.G[a], &r)) return false;
+    if (g.N[a] > g.G[a]) return false;
+  }
+  return true;
+}
+
+// Default choice: lines of at most 511 points, so that a y / z tile holds 16
+// lines (128-byte rows); longer lines leave half-line rows and rocFFT's
+// batched kernels are faster there.
+bool fused_fft_preferred(const Geom& g) {
+  if (!fused_fft_supported(g)) return false;
+  for (int a = 0; a < g.dim; a++)
+    if (16 * pad_of(g.G[a]) > kTileElems) return false;
+  return true;
+}
+
+size_t fused_fft_tmp_elems(const Geom& g) {
+  // adjoint z-pass / forward y-pass output (3-D), x-pass output (2-D)
+  return g.dim == 3 ? (size_t)g.G[0] * g.G[1] * g.N[2] : (size_t)g.G[0] * g.N[1];
+}
+
+// Device tables of the fused FFT, one float2 array: per axis and sign the
+// twiddles exp(sign 2 pi i k / G_a), k < G_a, computed in double
+// (offsets[a][0 | 1]), then per axis its Stockham stage table
+// (offsets[a][2], FftStage records; kernels read them with scalar loads).
+std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][3]) {
+  static_assert(sizeof(FftStage) % sizeof(float2) == 0, "stage records pack into float2 slots");
+  std::vector<float2> t;
+  for (int a = 0; a < 3; a++) offsets[a][0] = offsets[a][1] = offsets[a][2] = 0;
+  for (int a = 0; a < g.dim; a++)
+    for (int si = 0; si < 2; si++) {
+      offsets[a][si] = t.size();
+      const double sign = si == 0 ? -1.0 : 1.0;
+      const int L = g.G[a];
+      for (int k = 0; k < L; k++) {
+        const double ang = sign * 2.0 * M_PI * (double)k / (double)L;
+        t.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+      }
+    }
+  for (int a = 0; a < g.dim; a++) {
+    FftRadices r;
+    make_radices(g.G[a], &r);
+    offsets[a][2] = t.size();
+    const size_t slots = sizeof(r.st) / sizeof(float2);
+    t.resize(t.size() + slots);
+    std::memcpy(t.data() + offsets[a][2], r.st, sizeof(r.st));
+  }
+  return t;
+}
+
+// Adjoint: grid [coils][G2][G1][G0] (the INVERSE-shifted spectrum) -> image.
+// tmp holds coils * fused_fft_tmp_elems(g) elements; the grid is
+// overwritten. stop == 1 (FFT): per-coil images scaled by 1/sqrt(Ng) only.
+// accumulate: add the coil sum to `out` (later coil batches).
+hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,
+                             const float* deapo, const float2* sens, float2* out, int stop,
+                             bool accumulate, const float2* tw, const size_t tw_off[3][3],
+                             hipStream_t s) {
+  AxisPass ax[2];
+  LastPass last;
+  if (!build_adjoint(g, grid, tmp, coils, deapo, sens, out, stop, accumulate, tw, tw_off, ax,
+                     &last))
+    return hipErrorInvalidValue;
+  for (int i = 0; i < g.dim - 1; i++) {
+    hipError_t e = launch_axis(ax[i], s);
+    if (e != hipSuccess) return e;
+  }
+  const size_t lds = last_lds(last);
+  hipError_t e = allow_lds(fft_adjoint_last_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fft_adjoint_last_kernel, dim3(last_blocks(last)), dim3(kFftThreads), lds, s,
+                     last);
+  return hipGetLastError();
+}
+
+// Forward: image -> grid [coils][G2][G1][G0], the spectrum of the
+// zero-padded, INVERSE-shifted, deapodized (and coil-weighted) image. tmp
+// holds coils * fused_fft_tmp_elems(g) elements.
+hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil,
+                             const float2* sens, const float* deapo, float2* grid, float2* tmp,
+                             int coils, const float2* tw, const size_t tw_off[3][3],
+                             hipStream_t s) {
+  FirstPass first;
+  AxisPass ax[2];
+  if (!build_forward(g, img, img_per_coil, sens, deapo, grid, tmp, coils, tw, tw_off, &first, ax))
+    return hipErrorInvalidValue;
+  const size_t lds = first_lds(first);
+  hipError_t e = allow_lds(fft_forward_first_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fft_forward_first_kernel, dim3(first_blocks(first)), dim3(kFftThreads), lds,
+                     s, first);
+  e = hipGetLastError();
+  for (int i = 0; i < g.dim - 1 && e == hipSuccess; i++) e = launch_axis(ax[i], s);
+  return e;
+}
+
+// ---- batches --------------------------------------------------------------
+
+size_t fused_fft_desc_bytes() { return sizeof(FftOpDesc); }
+
+// Descriptor of one operator of a batch: its grid and intermediate, the
+// full adjoint (DEAPODIZATION stop, coil sum when sens) and forward. The
+// image pointers are set per call (fused_fft_desc_set_io).
+bool fused_fft_desc_build(void* desc, const Geom& g, float2* grid, float2* tmp, int coils,
+                          const float* deapo, const float2* sens, const float2* tw,
+                          const size_t tw_off[3][3]) {
+  FftOpDesc& d = *static_cast<FftOpDesc*>(desc);
+  d = FftOpDesc{};
+  d.n_ax = g.dim - 1;
+  return build_adjoint(g, grid, tmp, coils, deapo, sens, nullptr, 2, false, tw, tw_off, d.adj_ax,
+                       &d.adj_x) &&
+         build_forward(g, nullptr, sens == nullptr, sens, deapo, grid, tmp, coils, tw, tw_off,
+                       &d.fwd_x, d.fwd_ax);
+}
+
+void fused_fft_desc_set_io(void* desc, float2* adjoint_out, const float2* forward_img) {
+  FftOpDesc& d = *static_cast<FftOpDesc*>(desc);
+  d.adj_x.out = adjoint_out;
+  d.fwd_x.img = forward_img;
+}
+
+// Passes in launch order: adjoint ax[0 .. d-2], x; forward x, ax[0 .. d-2].
+int fused_fft_pass_count(int dim) { return dim; }
+
+uint32_t fused_fft_pass_blocks(const void* desc, bool adjoint, int pass) {
+  const FftOpDesc& d = *static_cast<const FftOpDesc*>(desc);
+  if (adjoint) return pass < d.n_ax ? axis_blocks(d.adj_ax[pass]) : last_blocks(d.adj_x);
+  return pass == 0 ? first_blocks(d.fwd_x) : axis_blocks(d.fwd_ax[pass - 1]);
+}
+
+size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass) {
+  const FftOpDesc& d = *static_cast<const FftOpDesc*>(desc);
+  if (adjoint) return pass < d.n_ax ? axis_lds(d.adj_ax[pass]) : last_lds(d.adj_x);
+  return pass == 0 ? first_lds(d.fwd_x) : axis_lds(d.fwd_ax[pass - 1]);
+}
+
+// One pass of every operator: items (operator, block), descs in device memory.
+hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, const void* descs_d,
+                                  const uint2* items, uint32_t n_items, size_t lds,
+                                  hipStream_t s) {
+  if (n_items == 0) return hipSuccess;
+  const FftOpDesc* d = static_cast<const FftOpDesc*>(descs_d);
+  const int n_ax = dim - 1;
+  hipError_t e = hipSuccess;
+  if (adjoint && pass == n_ax) {
+    e = allow_lds(fft_adjoint_last_batch_kernel, lds);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(fft_adjoint_last_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s,
+                         d, items);
+  } else if (!adjoint && pass == 0) {
+    e = allow_lds(fft_forward_first_batch_kernel, lds);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(fft_forward_first_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s,
+                         d, items);
+  } else {
+    e = allow_lds(fft_axis_batch_kernel, lds);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(fft_axis_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s, d, items,
+                         adjoint ? 1 : 0, adjoint ? pass : pass - 1);
+  }
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 }  // namespace gnufft

@@ -211,7 +211,7 @@ struct SpreadArgs {
   WorkList work;
   const uint32_t* centers; // [S][dim]
   const float* lut;
-  float2* scratch;         // [chunk][coil][padded point] sub-grids (n_chunks*n_coils*PD)
+  float2* scratch;         // [chunk][padded point][coil] sub-grids (n_chunks*PD*n_coils)
   const uint32_t* sector_chunks;  // [S+1] first chunk of each sector
   const uint2* reduce_tasks;      // (first chunk, count) of each multi-chunk sector
   uint32_t n_reduce;
@@ -295,7 +295,7 @@ struct BatchList {
 void partition_items(const uint32_t* costs, uint32_t n, BatchList* l);
 
 // gridding.hip (CB: coils per workgroup, the same for every operator)
-constexpr int kBatchCB = 4;
+constexpr int kBatchCB = 8;
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
 hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
@@ -317,15 +317,31 @@ hipError_t launch_pre_batch(const BatchOp* ops, const BatchList& l, hipStream_t 
 
 // Fused oversampled-grid FFT (fft.hip).
 bool fused_fft_supported(const Geom& g);
+bool fused_fft_preferred(const Geom& g);
 size_t fused_fft_tmp_elems(const Geom& g);
-std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][2]);
+std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][3]);
 hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,
                              const float* deapo, const float2* sens, float2* out, int stop,
-                             bool accumulate, const float2* tw, const size_t tw_off[3][2],
+                             bool accumulate, const float2* tw, const size_t tw_off[3][3],
                              hipStream_t s);
 hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil,
                              const float2* sens, const float* deapo, float2* grid, float2* tmp,
-                             int coils, const float2* tw, const size_t tw_off[3][2],
+                             int coils, const float2* tw, const size_t tw_off[3][3],
                              hipStream_t s);
+// Batches: one opaque descriptor (fused_fft_desc_bytes) per operator in
+// device memory; each pass of every operator is one launch over items
+// (operator, block). Passes: adjoint ax[0 .. d-2] then x, forward x then
+// ax[0 .. d-2] (fused_fft_pass_count(d) of each).
+size_t fused_fft_desc_bytes();
+bool fused_fft_desc_build(void* desc, const Geom& g, float2* grid, float2* tmp, int coils,
+                          const float* deapo, const float2* sens, const float2* tw,
+                          const size_t tw_off[3][3]);
+void fused_fft_desc_set_io(void* desc, float2* adjoint_out, const float2* forward_img);
+int fused_fft_pass_count(int dim);
+uint32_t fused_fft_pass_blocks(const void* desc, bool adjoint, int pass);
+size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass);
+hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, const void* descs_d,
+                                  const uint2* items, uint32_t n_items, size_t lds,
+                                  hipStream_t s);
 
 }  // namespace gnufft

@@ -300,7 +300,9 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
   for (int i = tid; i < L.sub_n; i += NT) sub[i] = 0ull;
 
-  // per-coil fixed-point scale from the chunk's L1 bound
+  // per-coil fixed-point scale from the chunk's L1 bound; the raw values of
+  // the first round's samples stay in sv (round 0 scales them in place
+  // instead of reading them again)
   {
     float l1[CB];
 #pragma unroll
@@ -314,6 +316,7 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
         if (coil0 + cc < a.n_coils) {
           const float2 v = a.data[(size_t)(coil0 + cc) * a.coil_stride + src];
           l1[cc] += fmaxf(fabsf(v.x), fabsf(v.y)) * sq;
+          if (i < R) sv[i * CB + cc] = v;
         }
     }
 #pragma unroll
@@ -346,7 +349,8 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
       for (int cc = 0; cc < CB; cc++) {
         float2 v = make_float2(0.0f, 0.0f);
         if (coil0 + cc < a.n_coils) {
-          v = a.data[(size_t)(coil0 + cc) * a.coil_stride + src];
+          // round 0: this thread staged sample i's raw value in the L1 pass
+          v = r0 == 0 ? sv[i * CB + cc] : a.data[(size_t)(coil0 + cc) * a.coil_stride + src];
           const double k = sq * scale_s[cc];
           v = make_float2((float)(v.x * k), (float)(v.y * k));
         }
@@ -400,9 +404,10 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
   }
   __syncthreads();
 
-  // Store the chunk's sub-grid, [coil][padded point], with plain coalesced
-  // stores; fold_kernel sums the sub-grids that overlap on the grid.
-  float2* dst = a.scratch + ((size_t)ci * a.n_coils + coil0) * PD;
+  // Store the chunk's sub-grid, [padded point][coil] like the LDS copy, with
+  // plain coalesced stores; fold_kernel sums the sub-grids that overlap on
+  // the grid (the coils of a point are one contiguous run for its reads).
+  float2* dst = a.scratch + (size_t)ci * a.n_coils * PD + coil0;
   for (int e = tid; e < PD * CB; e += NT) {
     const int pt = e / CB;
     const int cc = e % CB;
@@ -412,7 +417,7 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
     const int32_t im = (int32_t)(uint32_t)(u >> 32) - (re >> 31);
     const double sc = scale_s[cc];
     const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
-    dst[(size_t)cc * PD + pt] = make_float2((float)(re * inv), (float)(im * inv));
+    dst[(size_t)pt * a.n_coils + cc] = make_float2((float)(re * inv), (float)(im * inv));
   }
 }
 
@@ -523,7 +528,7 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
   }
   __syncthreads();
   const size_t kstride = (size_t)a.n_coils * PD;
-  const float2* scr = a.scratch + (size_t)coil0 * PD;
+  const float2* scr = a.scratch + coil0;  // [chunk][padded point][coil]
   for (int base = 0; base < HP; base += nt * kFoldPts) {
 #pragma unroll
     for (int p = 0; p < kFoldPts; p++) {
@@ -552,10 +557,11 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
             const uint32_t vx = cov[0][tx * kCovMax + ix];
             const int k = kofs[jyz * (int)ncx + (int)(vx & 255u)];
             if (k < 0) continue;
-            const float2* src = scr + (size_t)k * kstride + pyz + (int)(vx >> 8);
+            const float2* src =
+                scr + (size_t)k * kstride + (size_t)(pyz + (int)(vx >> 8)) * a.n_coils;
 #pragma unroll
             for (int c = 0; c < CB; c++) {
-              const float2 val = src[(size_t)min(c, a.n_coils - 1 - coil0) * PD];
+              const float2 val = src[min(c, a.n_coils - 1 - coil0)];
               ax_[c] += val.x;
               ay_[c] += val.y;
             }
@@ -678,7 +684,7 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
     for (int it = 0; it < n_items; it++) {
       const int4 v = items[it];
       const uint32_t rp = reps_of[it];
-      const float2* src = a.scratch + (size_t)v.w * kstride + (size_t)coil0 * PD;
+      const float2* src = a.scratch + (size_t)v.w * kstride + coil0;  // [chunk][point][coil]
 #pragma unroll
       for (int p = 0; p < kFoldPts; p++) {
         const int ix = wrap(q[p][0] - v.x, g.G[0]) + (int)(rp & 255u) * g.G[0];
@@ -689,7 +695,7 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
         const float m = in ? 1.0f : 0.0f;
 #pragma unroll
         for (int c = 0; c < CB; c++) {
-          const float2 val = src[(size_t)min(c, a.n_coils - 1 - coil0) * PD + pt];
+          const float2 val = src[(size_t)pt * a.n_coils + min(c, a.n_coils - 1 - coil0)];
           ax[p][c] += m * val.x;
           ay[p][c] += m * val.y;
         }

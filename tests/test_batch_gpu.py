@@ -1,7 +1,9 @@
 """Batches of independent plans (gnufft_batch_*): every plan's batched
 adjoint and forward equal its own single-plan adjoint and forward and the
 oracle (rel-L2 <= 1e-5), for uneven image sizes, trajectory lengths, coil
-counts, with and without sensitivities, in 2-D and 3-D."""
+counts, with and without sensitivities, in 2-D and 3-D, with the fused FFT
+(one launch per pass for all plans), with rocFFT (one plan per grid size)
+and with both in one batch."""
 
 import numpy as np
 import pytest
@@ -50,8 +52,9 @@ def _ops_3d():
     return out
 
 
+@pytest.mark.parametrize("fft", ["fused", "rocfft", "mixed"])
 @pytest.mark.parametrize("make", [_ops_2d, _ops_3d], ids=["2d", "3d"])
-def test_batch_equals_single_plans(oracle, env, make):
+def test_batch_equals_single_plans(oracle, env, make, fft):
     torch, capi = env
     specs = make()
     rng = np.random.default_rng(1)
@@ -61,6 +64,8 @@ def test_batch_equals_single_plans(oracle, env, make):
         p.set_trajectory(_dev(torch, sp["traj"]), _dev(torch, sp["dens"]))
         if sp["sens"] is not None:
             p.set_sens(_dev(torch, sp["sens"]), sp["C"])
+        fused = fft == "fused" or (fft == "mixed" and len(plans) % 2 == 0)
+        p.set_fft(capi.FFT_FUSED if fused else capi.FFT_ROCFFT)
         plans.append(p)
         M = sp["traj"].shape[1]
         ishape = tuple(sp["img"][::-1])
