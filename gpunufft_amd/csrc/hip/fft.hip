@@ -240,6 +240,46 @@ __device__ __forceinline__ void fft_lines(float2* buf, int lines, uint32_t m_lin
   }
 }
 
+// Compile-time stage tables for the common line lengths: every stage
+// constant (radix, Ns, strides, division constants) folds into the code.
+constexpr uint32_t qmagic(int d) {
+  return d == 1 ? 0u : (uint32_t)((0x100000000ull + (unsigned)d - 1) / (unsigned)d);
+}
+
+template <int L, int R0, int R1, int R2>
+struct FixedStages {
+  static_assert(R0 * R1 * R2 == L, "radices multiply to the line length");
+  static constexpr FftStage s0{R0, 1, L / R0, L / R0, 0u, 0u};
+  static constexpr FftStage s1{R1, R0, L / R1, L / (R0 * R1), qmagic(R0), 0u};
+  static constexpr FftStage s2{R2, R0 * R1, L / R2, 1, qmagic(R0 * R1), 0u};
+};
+
+// Stage runners: all stages of `lines` LDS lines (the x passes).
+struct RunTable {
+  const FftStage* st;
+  int n;
+  __device__ __forceinline__ void operator()(float2* buf, int lines, uint32_t m_lines, int LP,
+                                             const float2* tw, float sign) const {
+    fft_lines(buf, lines, m_lines, LP, st, n, tw, sign);
+  }
+};
+
+template <int L, int R0, int R1, int R2>
+struct RunFixed {
+  __device__ __forceinline__ void operator()(float2* buf, int lines, uint32_t m_lines, int LP,
+                                             const float2* tw, float sign) const {
+    using F = FixedStages<L, R0, R1, R2>;
+    auto get = [&](int l, int i) { return buf[l * LP + i]; };
+    auto put = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
+    stage<R0>(lines, m_lines, F::s0, tw, sign, true, get, put);
+    __syncthreads();
+    stage<R1>(lines, m_lines, F::s1, tw, sign, true, get, put);
+    __syncthreads();
+    stage<R2>(lines, m_lines, F::s2, tw, sign, true, get, put);
+    __syncthreads();
+  }
+};
+
 // ---------------------------------------------------------------------------
 // y / z pass over [coil][d2][d1][d0] arrays. Lines run along the pass axis;
 // a tile is `stripe` lines adjacent in x (a power of two, up to 32, that
@@ -324,6 +364,53 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
   fft_axis_body(p, blockIdx.x);
 }
 
+// The same pass for a line length L = R0 R1 R2 and tile of S lines known at
+// compile time (S a power of two).
+template <int L, int S, int R0, int R1, int R2>
+__device__ __forceinline__ void fft_axis_fixed_body(const AxisPass& p, int block) {
+  using F = FixedStages<L, R0, R1, R2>;
+  constexpr int LP = (L + 1) | 1;
+  constexpr uint32_t mS = qmagic(S);
+  extern __shared__ float2 lds[];
+  float2* tw = lds;
+  float2* buf = lds + L;
+  const int xt = block % p.x_tiles;
+  const int r = block / p.x_tiles;
+  const int ob = r % p.nob;
+  const int coil = r / p.nob;
+  const int x0 = xt * S;
+  const int nl = min(S, p.nx - x0);
+  for (int i = threadIdx.x; i < L; i += kFftThreads) tw[i] = p.tw[i];
+  const float2* in = p.in + coil * p.in_coil + ob * p.in_ob + x0;
+  float2* out = p.out + coil * p.out_coil + ob * p.out_ob + x0;
+  const uint32_t in_step = (uint32_t)p.in_step, out_step = (uint32_t)p.out_step;
+  auto get_global = [&](int l, int i) {
+    int idx = i;
+    if (p.in_scatter) {
+      idx = wrap1(i - p.in_off, L);
+      if (idx >= p.in_len) return make_float2(0.f, 0.f);
+    }
+    return l < nl ? in[(uint32_t)idx * in_step + (uint32_t)l] : make_float2(0.f, 0.f);
+  };
+  auto put_global = [&](int l, int o, float2 v) {
+    if (p.out_gather) o = wrap1(o - p.out_off, L);
+    if (l < nl && o < p.out_len) out[(uint32_t)o * out_step + (uint32_t)l] = v;
+  };
+  auto get_lds = [&](int l, int i) { return buf[l * LP + i]; };
+  auto put_lds = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
+  __syncthreads();  // twiddles
+  stage<R0>(S, mS, F::s0, tw, p.sign, false, get_global, put_lds);
+  __syncthreads();
+  stage<R1>(S, mS, F::s1, tw, p.sign, true, get_lds, put_lds);
+  __syncthreads();
+  stage<R2>(S, mS, F::s2, tw, p.sign, false, get_lds, put_global);
+}
+
+template <int L, int S, int R0, int R1, int R2>
+__global__ void __launch_bounds__(kFftThreads, 4) fft_axis_fixed_kernel(AxisPass p) {
+  fft_axis_fixed_body<L, S, R0, R1, R2>(p, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint last pass (x): lines [coil][N2][N1][G0]; pixel x of an image line
 // reads FFT index wrap(x + crop + shift_inv, G0), scaled by 1/sqrt(Ng)
@@ -349,7 +436,8 @@ struct LastPass {
   const float2* tw;
 };
 
-__device__ __forceinline__ void fft_adjoint_last_body(const LastPass& p, int block) {
+template <typename Run>
+__device__ __forceinline__ void fft_adjoint_last_body(const LastPass& p, int block, const Run& run) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.G0;
@@ -374,7 +462,7 @@ __device__ __forceinline__ void fft_adjoint_last_body(const LastPass& p, int blo
       }
     }
     __syncthreads();
-    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.st, p.n_st, tw, p.sign);
+    run(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, tw, p.sign);
     // the coil sum of later coil groups (and of later calls when
     // accumulating) adds to what this thread wrote for the same pixels
     const bool add = c0 > 0 || p.accumulate;
@@ -430,7 +518,8 @@ struct FirstPass {
   const float2* tw;
 };
 
-__device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int block) {
+template <typename Run>
+__device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int block, const Run& run) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
   float2* buf = lds + p.G0;
@@ -466,7 +555,7 @@ __device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int b
       }
     }
     __syncthreads();
-    fft_lines(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, p.st, p.n_st, tw, p.sign);
+    run(buf, ng * nq, qdiv_const_dev(ng * nq), p.LP, tw, p.sign);
     for (int c = 0; c < ng; c++) {
       float2* dst = p.out + (long long)(c0 + c) * cout + (long long)l0 * p.G0;
 #pragma unroll 4
@@ -480,11 +569,21 @@ __device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int b
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPass p) {
-  fft_adjoint_last_body(p, blockIdx.x);
+  fft_adjoint_last_body(p, blockIdx.x, RunTable{p.st, p.n_st});
+}
+
+template <int L, int R0, int R1, int R2>
+__global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_fixed_kernel(LastPass p) {
+  fft_adjoint_last_body(p, blockIdx.x, RunFixed<L, R0, R1, R2>{});
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(FirstPass p) {
-  fft_forward_first_body(p, blockIdx.x);
+  fft_forward_first_body(p, blockIdx.x, RunTable{p.st, p.n_st});
+}
+
+template <int L, int R0, int R1, int R2>
+__global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_fixed_kernel(FirstPass p) {
+  fft_forward_first_body(p, blockIdx.x, RunFixed<L, R0, R1, R2>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -515,14 +614,14 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_batch_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const LastPass p = d[it.x].adj_x;
-  fft_adjoint_last_body(p, (int)it.y);
+  fft_adjoint_last_body(p, (int)it.y, RunTable{p.st, p.n_st});
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_batch_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const FirstPass p = d[it.x].fwd_x;
-  fft_forward_first_body(p, (int)it.y);
+  fft_forward_first_body(p, (int)it.y, RunTable{p.st, p.n_st});
 }
 
 // ---------------------------------------------------------------------------
@@ -605,12 +704,61 @@ size_t last_lds(const LastPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
 uint32_t first_blocks(const FirstPass& p) { return (uint32_t)((p.lines + p.Q - 1) / p.Q); }
 size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
 
+// Line lengths with compile-time kernels: (L, tile lines of a y / z pass,
+// radices). The tile must be the one finish_axis picks for L.
+#define GNUFFT_FIXED_FFT_SIZES(X) \
+  X(96, 32, 8, 4, 3)              \
+  X(128, 32, 8, 8, 2)             \
+  X(160, 32, 8, 4, 5)             \
+  X(192, 32, 8, 8, 3)             \
+  X(256, 16, 8, 8, 4)
+
+template <typename K>
+hipError_t launch_pass(K kernel, uint32_t blocks, size_t lds, hipStream_t s,
+                       const AxisPass& p) {
+  hipError_t e = allow_lds(kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kFftThreads), lds, s, p);
+  return hipGetLastError();
+}
+
 hipError_t launch_axis(const AxisPass& p, hipStream_t s) {
   const size_t lds = axis_lds(p);
-  hipError_t e = allow_lds(fft_axis_kernel, lds);
+  const uint32_t blocks = axis_blocks(p);
+#define GNUFFT_AXIS_CASE(LEN, TILE, RA, RB, RC)    \
+  if (p.L == LEN && p.stripe == TILE)              \
+    return launch_pass(fft_axis_fixed_kernel<LEN, TILE, RA, RB, RC>, blocks, lds, s, p);
+  GNUFFT_FIXED_FFT_SIZES(GNUFFT_AXIS_CASE)
+#undef GNUFFT_AXIS_CASE
+  return launch_pass(fft_axis_kernel, blocks, lds, s, p);
+}
+
+template <typename K, typename P>
+hipError_t launch_x(K kernel, uint32_t blocks, size_t lds, hipStream_t s, const P& p) {
+  hipError_t e = allow_lds(kernel, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fft_axis_kernel, dim3(axis_blocks(p)), dim3(kFftThreads), lds, s, p);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kFftThreads), lds, s, p);
   return hipGetLastError();
+}
+
+hipError_t launch_last(const LastPass& p, hipStream_t s) {
+  const size_t lds = last_lds(p);
+  const uint32_t blocks = last_blocks(p);
+#define GNUFFT_LAST_CASE(LEN, TILE, RA, RB, RC) \
+  if (p.G0 == LEN) return launch_x(fft_adjoint_last_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
+  GNUFFT_FIXED_FFT_SIZES(GNUFFT_LAST_CASE)
+#undef GNUFFT_LAST_CASE
+  return launch_x(fft_adjoint_last_kernel, blocks, lds, s, p);
+}
+
+hipError_t launch_first(const FirstPass& p, hipStream_t s) {
+  const size_t lds = first_lds(p);
+  const uint32_t blocks = first_blocks(p);
+#define GNUFFT_FIRST_CASE(LEN, TILE, RA, RB, RC) \
+  if (p.G0 == LEN) return launch_x(fft_forward_first_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
+  GNUFFT_FIXED_FFT_SIZES(GNUFFT_FIRST_CASE)
+#undef GNUFFT_FIRST_CASE
+  return launch_x(fft_forward_first_kernel, blocks, lds, s, p);
 }
 
 // Pass descriptors of the adjoint: grid [coils][G2][G1][G0] (the
@@ -843,12 +991,7 @@ hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils
     hipError_t e = launch_axis(ax[i], s);
     if (e != hipSuccess) return e;
   }
-  const size_t lds = last_lds(last);
-  hipError_t e = allow_lds(fft_adjoint_last_kernel, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fft_adjoint_last_kernel, dim3(last_blocks(last)), dim3(kFftThreads), lds, s,
-                     last);
-  return hipGetLastError();
+  return launch_last(last, s);
 }
 
 // Forward: image -> grid [coils][G2][G1][G0], the spectrum of the
@@ -862,12 +1005,7 @@ hipError_t fused_fft_forward(const Geom& g, const float2* img, bool img_per_coil
   AxisPass ax[2];
   if (!build_forward(g, img, img_per_coil, sens, deapo, grid, tmp, coils, tw, tw_off, &first, ax))
     return hipErrorInvalidValue;
-  const size_t lds = first_lds(first);
-  hipError_t e = allow_lds(fft_forward_first_kernel, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fft_forward_first_kernel, dim3(first_blocks(first)), dim3(kFftThreads), lds,
-                     s, first);
-  e = hipGetLastError();
+  hipError_t e = launch_first(first, s);
   for (int i = 0; i < g.dim - 1 && e == hipSuccess; i++) e = launch_axis(ax[i], s);
   return e;
 }

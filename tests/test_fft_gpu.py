@@ -7,8 +7,9 @@ deapodization -> coil sum (adjoint) and coil weight -> deapodization -> pad
 full adjoint (stops FFT and DEAPODIZATION, with and without sensitivities)
 and forward on one plan with both FFT kinds; both must equal the oracle
 (rel-L2 <= 1e-5) and each other. The grid lengths cover even and odd
-lengths, radix 2/3/4/5/8 mixes, anisotropic 3-D grids and a length with a
-factor 7 (rocFFT only: set_fft(FUSED) is refused).
+lengths, radix 2/3/4/5/8 mixes, the lengths with compile-time stage tables
+(96, 128, 160, 192, 256), anisotropic 3-D grids and a length with a factor 7
+(rocFFT only: set_fft(FUSED) is refused).
 """
 
 import numpy as np
@@ -24,7 +25,10 @@ CASES = [
     ((32, 32), 1.5),        # 48 = 8 * 2 * 3
     ((30, 30), 1.5),        # 45 = 3 * 3 * 5 (odd)
     ((50, 40), 1.5),        # 75 x 60
-    ((64, 64), 2.0),        # 128
+    ((64, 64), 2.0),        # 128 (compile-time stages)
+    ((64, 48), 1.5),        # 96 x 72 (96: compile-time stages)
+    ((80, 80), 2.0),        # 160 (compile-time stages, radix 5)
+    ((128, 96), 2.0),       # 256 x 192 (compile-time stages, 16-line tiles)
     ((20, 24, 16), 1.5),    # 30 x 36 x 24
     ((16, 16, 16), 2.0),    # 32^3
     ((25, 16, 10), 2.0),    # 50 x 32 x 20
