@@ -529,6 +529,9 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
   __syncthreads();
   const size_t kstride = (size_t)a.n_coils * PD;
   const float2* scr = a.scratch + coil0;  // [chunk][padded point][coil]
+  // full block of an even coil count: every point's CB coils start on a
+  // 16-byte boundary
+  const bool vec = CB % 2 == 0 && coil0 + CB <= a.n_coils && a.n_coils % 2 == 0;
   for (int base = 0; base < HP; base += nt * kFoldPts) {
 #pragma unroll
     for (int p = 0; p < kFoldPts; p++) {
@@ -559,11 +562,24 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
             if (k < 0) continue;
             const float2* src =
                 scr + (size_t)k * kstride + (size_t)(pyz + (int)(vx >> 8)) * a.n_coils;
+            if (vec) {
+              // the block's CB coils: CB/2 16-byte loads (16-byte aligned)
+              const float4* s4 = reinterpret_cast<const float4*>(src);
 #pragma unroll
-            for (int c = 0; c < CB; c++) {
-              const float2 val = src[min(c, a.n_coils - 1 - coil0)];
-              ax_[c] += val.x;
-              ay_[c] += val.y;
+              for (int c2 = 0; c2 < CB / 2; c2++) {
+                const float4 val = s4[c2];
+                ax_[2 * c2] += val.x;
+                ay_[2 * c2] += val.y;
+                ax_[2 * c2 + 1] += val.z;
+                ay_[2 * c2 + 1] += val.w;
+              }
+            } else {
+#pragma unroll
+              for (int c = 0; c < CB; c++) {
+                const float2 val = src[min(c, a.n_coils - 1 - coil0)];
+                ax_[c] += val.x;
+                ay_[c] += val.y;
+              }
             }
           }
         }
