@@ -1243,7 +1243,19 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     if (ga.G[1] != gc.G[1]) return ga.G[1] < gc.G[1];
     return ga.G[0] < gc.G[0];
   });
-  const size_t half = lds_budget() / 2 - 1024;
+  // One launch per stage uses the largest footprint of any plan, so each
+  // plan's round size is the largest that still lets four workgroups share
+  // a CU (then three, two, one): one dense plan must not cut the occupancy
+  // of the whole batch.
+  auto batch_round = [](int (*round)(const Geom&, int, int, size_t), const Geom& g, int target,
+                        size_t reserve) {
+    const int need = std::min(64, target);
+    for (int k = 4; k >= 1; k--) {
+      const int r = round(g, kBatchCB, target, lds_budget() / k - 1024 - reserve);
+      if (r >= need || (k == 1 && r > 0)) return r;
+    }
+    return 0;
+  };
   size_t grid_total = 0, scratch_total = 0;
   std::vector<size_t> grid_off(n_plans), scratch_off(n_plans);
   for (uint32_t k = 0; k < n_plans; k++) {
@@ -1304,8 +1316,8 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
                        p->n_sens ? p->sens_d : nullptr, nullptr, GNUFFT_STOP_DEAPODIZATION};
     op.pre = PreArgs{nullptr, p->n_sens ? 0 : 1, p->n_sens ? p->sens_d : nullptr, p->deapo_d,
                      b->grid_d + grid_off[k], (int)C};
-    op.R_spread = spread_round(p->g, kBatchCB, (int)p->round_target, half);
-    op.R_interp = interp_round(p->g, kBatchCB, (int)p->round_target, half - 3072);
+    op.R_spread = batch_round(spread_round, p->g, (int)p->round_target, 0);
+    op.R_interp = batch_round(interp_round, p->g, (int)p->round_target, 3072);
     if (op.R_spread <= 0 || op.R_interp <= 0) {
       rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: padded sector does not fit in LDS");
       break;
