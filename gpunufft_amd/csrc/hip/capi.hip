@@ -1171,7 +1171,8 @@ struct gnufft_batch_s {
     uint32_t n = 0;
     size_t lds = 0;
   };
-  FftPass fft_adj[3], fft_fwd[3];
+  FftPass fft_adj[3], fft_fwd[3];          // passes on the standard tiles
+  FftPass fft_adj_big[3], fft_fwd_big[3];  // y / z passes of long lines (large tiles)
   hipEvent_t uploaded = nullptr;  // last descriptor upload (ops_h may be reused after it)
 };
 
@@ -1409,18 +1410,21 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
       std::vector<uint2> fi;
       std::vector<size_t> first;
       for (int dir = 0; dir < 2; dir++)
-        for (int ps = 0; ps < np; ps++) {
-          gnufft_batch_s::FftPass& fp = dir == 0 ? b->fft_adj[ps] : b->fft_fwd[ps];
-          first.push_back(fi.size());
-          for (uint32_t k = 0; k < n_plans; k++) {
-            if (!b->op_fused[k]) continue;
-            const void* d = b->fft_desc_h.data() + k * db;
-            const uint32_t nb = fused_fft_pass_blocks(d, dir == 0, ps);
-            for (uint32_t blk = 0; blk < nb; blk++) fi.push_back(make_uint2(k, blk));
-            fp.lds = std::max(fp.lds, fused_fft_pass_lds(d, dir == 0, ps));
+        for (int ps = 0; ps < np; ps++)
+          for (int big = 0; big < 2; big++) {
+            gnufft_batch_s::FftPass& fp = dir == 0 ? (big ? b->fft_adj_big[ps] : b->fft_adj[ps])
+                                                   : (big ? b->fft_fwd_big[ps] : b->fft_fwd[ps]);
+            first.push_back(fi.size());
+            for (uint32_t k = 0; k < n_plans; k++) {
+              if (!b->op_fused[k]) continue;
+              const void* d = b->fft_desc_h.data() + k * db;
+              if (fused_fft_pass_big(d, dir == 0, ps) != big) continue;
+              const uint32_t nb = fused_fft_pass_blocks(d, dir == 0, ps);
+              for (uint32_t blk = 0; blk < nb; blk++) fi.push_back(make_uint2(k, blk));
+              fp.lds = std::max(fp.lds, fused_fft_pass_lds(d, dir == 0, ps));
+            }
+            fp.n = (uint32_t)(fi.size() - first.back());
           }
-          fp.n = (uint32_t)(fi.size() - first.back());
-        }
       if (hipMalloc(&b->fft_desc_d, std::max<size_t>(b->fft_desc_h.size(), 1)) != hipSuccess ||
           hipMalloc(&b->fft_items_d, std::max<size_t>(fi.size(), 1) * sizeof(uint2)) != hipSuccess) {
         rc = fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: FFT descriptor allocation");
@@ -1431,7 +1435,10 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
         size_t j = 0;
         for (int dir = 0; dir < 2; dir++)
           for (int ps = 0; ps < np; ps++)
-            (dir == 0 ? b->fft_adj[ps] : b->fft_fwd[ps]).items = b->fft_items_d + first[j++];
+            for (int big = 0; big < 2; big++)
+              (dir == 0 ? (big ? b->fft_adj_big[ps] : b->fft_adj[ps])
+                        : (big ? b->fft_fwd_big[ps] : b->fft_fwd[ps]))
+                  .items = b->fft_items_d + first[j++];
       }
     }
     for (auto& gr : b->groups) {
@@ -1477,9 +1484,13 @@ int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* 
   HIP_TRY(launch_reduce_batch(b->ops_d, b->reduce, b->max_kstride, s));
   HIP_TRY(launch_fold_batch(b->dim, b->ops_d, b->fold, b->fold_threads, s));
   if (b->fused) {
-    for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++)
-      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, b->fft_desc_d, b->fft_adj[ps].items,
+    for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++) {
+      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, false, b->fft_desc_d, b->fft_adj[ps].items,
                                      b->fft_adj[ps].n, b->fft_adj[ps].lds, s));
+      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, true, b->fft_desc_d,
+                                     b->fft_adj_big[ps].items, b->fft_adj_big[ps].n,
+                                     b->fft_adj_big[ps].lds, s));
+    }
   }
   for (auto& gr : b->groups) {
     rc = exec_fft(gr.fft, true, b->grid_d + gr.offset, s);
@@ -1511,9 +1522,13 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
     if (rc) return rc;
   }
   if (b->fused) {
-    for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++)
-      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, b->fft_desc_d, b->fft_fwd[ps].items,
+    for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++) {
+      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, false, b->fft_desc_d, b->fft_fwd[ps].items,
                                      b->fft_fwd[ps].n, b->fft_fwd[ps].lds, s));
+      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, true, b->fft_desc_d,
+                                     b->fft_fwd_big[ps].items, b->fft_fwd_big[ps].n,
+                                     b->fft_fwd_big[ps].lds, s));
+    }
   }
   HIP_TRY(launch_interp_batch(b->dim, b->nx, b->ops_d, b->interp, b->interp_lds, s));
   return GNUFFT_OK;

@@ -36,6 +36,11 @@ namespace {
 
 constexpr int kFftThreads = 512;
 constexpr int kTileElems = 8192;  // complex elements per LDS tile (64 KB)
+constexpr int kPerThread = kTileElems / kFftThreads;  // tile elements per thread
+// y / z passes of lines longer than 511 points: 1024 threads and a 128 KB
+// tile of 16 lines (one workgroup per CU), the same work per thread
+constexpr int kBigThreads = 1024;
+constexpr int kBigTileElems = kPerThread * kBigThreads;
 constexpr int kMaxStripe = 32;    // lines per tile of a y / z pass (256 B rows)
 constexpr int kMaxPix = 4;        // image pixels per thread in the x passes
 constexpr int kMaxLen = kMaxPix * kFftThreads;  // longest transform (2048)
@@ -161,7 +166,7 @@ struct FftRadices {
 
 template <int R>
 struct Bfly {
-  static constexpr int KB = (kTileElems / kFftThreads + R - 1) / R;  // butterflies per thread
+  static constexpr int KB = (kPerThread + R - 1) / R;  // butterflies per thread
 };
 
 // Twiddle, radix-R DFT and store of butterfly j (values v) at
@@ -185,7 +190,7 @@ __device__ __forceinline__ void finish_butterfly(float2* v, int j, int l, const 
 // Stage with its inputs fetched by get(l, index) and outputs stored by
 // put(l, index, value); a barrier separates all fetches from all stores
 // when `sync` (in-place LDS stages).
-template <int R, typename Get, typename Put>
+template <int R, int NT = kFftThreads, typename Get, typename Put>
 __device__ __forceinline__ void stage(int lines, uint32_t m_lines, const FftStage& st, const float2* tw,
                              float sign, bool sync, Get get, Put put) {
   constexpr int KB = Bfly<R>::KB;
@@ -193,7 +198,7 @@ __device__ __forceinline__ void stage(int lines, uint32_t m_lines, const FftStag
   float2 v[KB][R];
 #pragma unroll
   for (int q = 0; q < KB; q++) {
-    const int b = threadIdx.x + q * kFftThreads;
+    const int b = threadIdx.x + q * NT;
     if (b < B) {
       const int j = qdiv(b, m_lines);
       const int l = b - j * lines;
@@ -204,7 +209,7 @@ __device__ __forceinline__ void stage(int lines, uint32_t m_lines, const FftStag
   if (sync) __syncthreads();
 #pragma unroll
   for (int q = 0; q < KB; q++) {
-    const int b = threadIdx.x + q * kFftThreads;
+    const int b = threadIdx.x + q * NT;
     if (b < B) {
       const int j = qdiv(b, m_lines);
       const int l = b - j * lines;
@@ -213,18 +218,18 @@ __device__ __forceinline__ void stage(int lines, uint32_t m_lines, const FftStag
   }
 }
 
-template <typename Get, typename Put>
+template <int NT = kFftThreads, typename Get, typename Put>
 __device__ __forceinline__ void stage_any(int lines, uint32_t m_lines, const FftStage& table_entry,
                                  const float2* tw, float sign, bool sync, Get get, Put put) {
   // a register copy: the table lives in global memory, and stores of the
   // stage to global memory would otherwise force reloads of its fields
   const FftStage st = table_entry;
   switch (st.R) {
-    case 8: stage<8>(lines, m_lines, st, tw, sign, sync, get, put); break;
-    case 4: stage<4>(lines, m_lines, st, tw, sign, sync, get, put); break;
-    case 2: stage<2>(lines, m_lines, st, tw, sign, sync, get, put); break;
-    case 3: stage<3>(lines, m_lines, st, tw, sign, sync, get, put); break;
-    default: stage<5>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 8: stage<8, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 4: stage<4, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 2: stage<2, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 3: stage<3, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    default: stage<5, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
   }
 }
 
@@ -304,6 +309,7 @@ struct AxisPass {
   int in_scatter, in_off;
   int out_gather, out_off;
   int stripe, x_tiles, coils;
+  int big;  // 1: lines above 511 points, kBigThreads and a kBigTileElems tile
   uint32_t m_stripe;
   float sign;
   const FftStage* st;  // stage table in device memory (beside the twiddles)
@@ -311,6 +317,7 @@ struct AxisPass {
   const float2* tw;
 };
 
+template <int NT>
 __device__ __forceinline__ void fft_axis_body(const AxisPass& p, int block) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
@@ -322,7 +329,7 @@ __device__ __forceinline__ void fft_axis_body(const AxisPass& p, int block) {
   const int coil = r / p.nob;
   const int x0 = xt * p.stripe;
   const int nl = min(p.stripe, p.nx - x0);
-  for (int i = threadIdx.x; i < p.L; i += kFftThreads) tw[i] = p.tw[i];
+  for (int i = threadIdx.x; i < p.L; i += NT) tw[i] = p.tw[i];
   const float2* in = p.in + coil * p.in_coil + ob * p.in_ob + x0;
   float2* out = p.out + coil * p.out_coil + ob * p.out_ob + x0;
   const int L = p.L, LP = p.LP;
@@ -348,20 +355,24 @@ __device__ __forceinline__ void fft_axis_body(const AxisPass& p, int block) {
   const int n = p.n_st;
   __syncthreads();  // twiddles
   if (n == 1) {
-    stage_any(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_global);
+    stage_any<NT>(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_global);
     return;
   }
-  stage_any(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_lds);
+  stage_any<NT>(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_lds);
   __syncthreads();
   for (int i = 1; i < n - 1; i++) {
-    stage_any(S, p.m_stripe, p.st[i], tw, p.sign, true, get_lds, put_lds);
+    stage_any<NT>(S, p.m_stripe, p.st[i], tw, p.sign, true, get_lds, put_lds);
     __syncthreads();
   }
-  stage_any(S, p.m_stripe, p.st[n - 1], tw, p.sign, false, get_lds, put_global);
+  stage_any<NT>(S, p.m_stripe, p.st[n - 1], tw, p.sign, false, get_lds, put_global);
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
-  fft_axis_body(p, blockIdx.x);
+  fft_axis_body<kFftThreads>(p, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBigThreads, 1) fft_axis_big_kernel(AxisPass p) {
+  fft_axis_body<kBigThreads>(p, blockIdx.x);
 }
 
 // The same pass for a line length L = R0 R1 R2 and tile of S lines known at
@@ -607,7 +618,15 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_batch_kernel(const Ff
   // register copies: the descriptors live in global memory, which the
   // passes also store to
   const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
-  fft_axis_body(p, (int)it.y);
+  fft_axis_body<kFftThreads>(p, (int)it.y);
+}
+
+__global__ void __launch_bounds__(kBigThreads, 1) fft_axis_batch_big_kernel(
+    const FftOpDesc* __restrict__ d, const uint2* __restrict__ items, int adjoint, int which) {
+  const uint2 it = items[blockIdx.x];
+  const FftOpDesc& o = d[it.x];
+  const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
+  fft_axis_body<kBigThreads>(p, (int)it.y);
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_batch_kernel(
@@ -689,8 +708,12 @@ bool finish_axis(AxisPass* p, int coils, const FftStage* st) {
   if (!make_radices(p->L, &r)) return false;
   p->st = st;
   p->n_st = r.n;
+  // 16-line tiles at least (128-byte rows): lines above 511 points take the
+  // large tile
+  p->big = 16 * p->LP > kTileElems ? 1 : 0;
+  const int tile = p->big ? kBigTileElems : kTileElems;
   int lg = 0;
-  while ((2 << lg) <= kMaxStripe && (2 << lg) * p->LP <= kTileElems) lg++;
+  while ((2 << lg) <= kMaxStripe && (2 << lg) * p->LP <= tile) lg++;
   p->stripe = 1 << lg;
   p->m_stripe = qdiv_const(p->stripe);
   p->x_tiles = (p->nx + p->stripe - 1) / p->stripe;
@@ -715,10 +738,10 @@ size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
 
 template <typename K>
 hipError_t launch_pass(K kernel, uint32_t blocks, size_t lds, hipStream_t s,
-                       const AxisPass& p) {
+                       const AxisPass& p, int threads = kFftThreads) {
   hipError_t e = allow_lds(kernel, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kFftThreads), lds, s, p);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), lds, s, p);
   return hipGetLastError();
 }
 
@@ -730,6 +753,7 @@ hipError_t launch_axis(const AxisPass& p, hipStream_t s) {
     return launch_pass(fft_axis_fixed_kernel<LEN, TILE, RA, RB, RC>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_AXIS_CASE)
 #undef GNUFFT_AXIS_CASE
+  if (p.big) return launch_pass(fft_axis_big_kernel, blocks, lds, s, p, kBigThreads);
   return launch_pass(fft_axis_kernel, blocks, lds, s, p);
 }
 
@@ -930,13 +954,13 @@ bool fused_fft_supported(const Geom& g) {
   return true;
 }
 
-// Default choice: lines of at most 511 points, so that a y / z tile holds 16
-// lines (128-byte rows); longer lines leave half-line rows and rocFFT's
-// batched kernels are faster there.
+// Default choice: lines of at most 1023 points, so that a y / z tile holds
+// at least 16 lines (128-byte rows); longer lines leave shorter rows and
+// rocFFT's kernels are faster there.
 bool fused_fft_preferred(const Geom& g) {
   if (!fused_fft_supported(g)) return false;
   for (int a = 0; a < g.dim; a++)
-    if (16 * pad_of(g.G[a]) > kTileElems) return false;
+    if (16 * pad_of(g.G[a]) > kBigTileElems) return false;
   return true;
 }
 
@@ -1044,6 +1068,14 @@ uint32_t fused_fft_pass_blocks(const void* desc, bool adjoint, int pass) {
   return pass == 0 ? first_blocks(d.fwd_x) : axis_blocks(d.fwd_ax[pass - 1]);
 }
 
+// 1 when the pass of this operator runs on the large-tile kernel (a y / z
+// pass of lines above 511 points)
+int fused_fft_pass_big(const void* desc, bool adjoint, int pass) {
+  const FftOpDesc& d = *static_cast<const FftOpDesc*>(desc);
+  if (adjoint) return pass < d.n_ax ? d.adj_ax[pass].big : 0;
+  return pass == 0 ? 0 : d.fwd_ax[pass - 1].big;
+}
+
 size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass) {
   const FftOpDesc& d = *static_cast<const FftOpDesc*>(desc);
   if (adjoint) return pass < d.n_ax ? axis_lds(d.adj_ax[pass]) : last_lds(d.adj_x);
@@ -1051,7 +1083,7 @@ size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass) {
 }
 
 // One pass of every operator: items (operator, block), descs in device memory.
-hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, const void* descs_d,
+hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, bool big, const void* descs_d,
                                   const uint2* items, uint32_t n_items, size_t lds,
                                   hipStream_t s) {
   if (n_items == 0) return hipSuccess;
@@ -1068,6 +1100,11 @@ hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, const void* d
     if (e == hipSuccess)
       hipLaunchKernelGGL(fft_forward_first_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s,
                          d, items);
+  } else if (big) {
+    e = allow_lds(fft_axis_batch_big_kernel, lds);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(fft_axis_batch_big_kernel, dim3(n_items), dim3(kBigThreads), lds, s, d,
+                         items, adjoint ? 1 : 0, adjoint ? pass : pass - 1);
   } else {
     e = allow_lds(fft_axis_batch_kernel, lds);
     if (e == hipSuccess)

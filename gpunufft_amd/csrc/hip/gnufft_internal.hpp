@@ -340,7 +340,8 @@ void fused_fft_desc_set_io(void* desc, float2* adjoint_out, const float2* forwar
 int fused_fft_pass_count(int dim);
 uint32_t fused_fft_pass_blocks(const void* desc, bool adjoint, int pass);
 size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass);
-hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, const void* descs_d,
+int fused_fft_pass_big(const void* desc, bool adjoint, int pass);
+hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, bool big, const void* descs_d,
                                   const uint2* items, uint32_t n_items, size_t lds,
                                   hipStream_t s);
 

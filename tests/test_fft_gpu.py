@@ -8,8 +8,9 @@ full adjoint (stops FFT and DEAPODIZATION, with and without sensitivities)
 and forward on one plan with both FFT kinds; both must equal the oracle
 (rel-L2 <= 1e-5) and each other. The grid lengths cover even and odd
 lengths, radix 2/3/4/5/8 mixes, the lengths with compile-time stage tables
-(96, 128, 160, 192, 256), anisotropic 3-D grids and a length with a factor 7
-(rocFFT only: set_fft(FUSED) is refused).
+(96, 128, 160, 192, 256), lines above 511 points (large LDS tiles),
+anisotropic 3-D grids and a length with a factor 7 (rocFFT only:
+set_fft(FUSED) is refused).
 """
 
 import numpy as np
@@ -29,6 +30,8 @@ CASES = [
     ((64, 48), 1.5),        # 96 x 72 (96: compile-time stages)
     ((80, 80), 2.0),        # 160 (compile-time stages, radix 5)
     ((128, 96), 2.0),       # 256 x 192 (compile-time stages, 16-line tiles)
+    ((256, 160), 2.0),      # 512 x 320 (large tiles along y: 1024 threads)
+    ((320, 384), 2.0),      # 640 x 768 (large tiles, radix 5 / 3)
     ((20, 24, 16), 1.5),    # 30 x 36 x 24
     ((16, 16, 16), 2.0),    # 32^3
     ((25, 16, 10), 2.0),    # 50 x 32 x 20
