@@ -148,8 +148,8 @@ def main():
     achieved = spread_bytes / (spread_ms * 1e-3) / 1e9
     traffic = traffic_summary()
     roofline = {
-        "kernel": "adjoint gridding stage: spread_kernel<3,4,8> + reduce_chunks_kernel + "
-                  "fold_kernel<3,8>",
+        "kernel": "adjoint gridding stage: spread_kernel + reduce_chunks_kernel + fold_kernel "
+                  "(3-D, 4-tap rows; coil blocks as launched)",
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
