@@ -90,10 +90,16 @@ __device__ inline bool batch_item(const BatchList& l, uint32_t bid, uint4* it) {
 static inline uint32_t batch_blocks(const BatchList& l) { return 8u * (l.max_range ? l.max_range : 1u); }
 
 template <int DIM>
-__device__ inline void load_center(const uint32_t* centers, uint32_t sector, int* c) {
-  c[0] = (int)centers[DIM * sector];
-  c[1] = (int)centers[DIM * sector + 1];
-  c[2] = DIM == 3 ? (int)centers[DIM * sector + 2] : 0;
+__device__ inline void load_center(const Geom& g, uint32_t sector, int* c) {
+  // the standard center layout (index * sw + sw / 2 per axis, the only one
+  // plans accept, gpuNUFFT_operator_factory.cpp:207-214): computed, not
+  // loaded, which saves a dependent memory round trip at workgroup start
+  const uint32_t x = sector % (uint32_t)g.S[0];
+  const uint32_t yz = sector / (uint32_t)g.S[0];
+  const int half = g.sw / 2;
+  c[0] = (int)x * g.sw + half;
+  c[1] = (int)(yz % (uint32_t)g.S[1]) * g.sw + half;
+  c[2] = DIM == 3 ? (int)(yz / (uint32_t)g.S[1]) * g.sw + half : 0;
 }
 
 // Global grid index of padded-sector point pt, with periodic wrap and an
@@ -294,7 +300,7 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
   const int lane = tid & 63;
   const int wave = tid >> 6;
   int c3[3];
-  load_center<DIM>(a.centers, ch.sector, c3);
+  load_center<DIM>(g, ch.sector, c3);
 
   for (int i = tid; i < g.K; i += NT) lut[i] = a.lut[i];
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
@@ -827,7 +833,7 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   const int coil0 = coil_block * CB;
   const int tid = threadIdx.x;
   int c3[3];
-  load_center<DIM>(a.centers, ch.sector, c3);
+  load_center<DIM>(g, ch.sector, c3);
 
   // grid offset of padded index i along axis d (periodic wrap, FORWARD
   // fftshift folded in): the padded point (x, y, z) reads grid element
