@@ -268,6 +268,20 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
+// Copy n floats to LDS with U loads in flight per thread (a plain loop
+// waits for each load before its LDS write).
+template <int NT, int U>
+__device__ inline void copy_to_lds(float* dst, const float* __restrict__ src, int n) {
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+    float t[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) t[u] = i0 + u * NT < n ? src[i0 + u * NT] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + u * NT < n) dst[i0 + u * NT] = t[u];
+  }
+}
+
 // One workgroup's share of the spread: chunk ci of the work list, coils
 // [coil_block * CB, coil_block * CB + CB).
 template <int DIM, int NX, int CB>
@@ -281,6 +295,7 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
   extern __shared__ float4 smem4[];
   __shared__ float l1_part[NW][CB];
   __shared__ double scale_s[CB];
+  __shared__ double inv_s[CB];
   char* smem = reinterpret_cast<char*>(smem4);
   const SpreadLayout L(g, NX, CB, R);
   float* lut = reinterpret_cast<float*>(smem);
@@ -302,9 +317,13 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
   int c3[3];
   load_center<DIM>(g, ch.sector, c3);
 
-  for (int i = tid; i < g.K; i += NT) lut[i] = a.lut[i];
+  copy_to_lds<NT, 4>(lut, a.lut, g.K);
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
-  for (int i = tid; i < L.sub_n; i += NT) sub[i] = 0ull;
+  {
+    // 16-byte stores; the allocation rounds sub_n up to an even count
+    ulonglong2* sub2 = reinterpret_cast<ulonglong2*>(sub);
+    for (int i = tid; i < (L.sub_n + 1) / 2; i += NT) sub2[i] = make_ulonglong2(0ull, 0ull);
+  }
 
   // per-coil fixed-point scale from the chunk's L1 bound; the raw values of
   // the first round's samples stay in sv (round 0 scales them in place
@@ -334,7 +353,9 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
     if (tid < CB) {
       double t = 0.0;
       for (int w = 0; w < NW; w++) t += (double)l1_part[w][tid];
-      scale_s[tid] = t > 0.0 ? kFixedOne / t : 0.0;
+      const double sc = t > 0.0 ? kFixedOne / t : 0.0;
+      scale_s[tid] = sc;
+      inv_s[tid] = sc > 0.0 ? 1.0 / sc : 0.0;
     }
   }
 
@@ -421,8 +442,7 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
     const unsigned long long u = sub[e];
     const int32_t re = (int32_t)(uint32_t)u;
     const int32_t im = (int32_t)(uint32_t)(u >> 32) - (re >> 31);
-    const double sc = scale_s[cc];
-    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+    const double inv = inv_s[cc];
     dst[(size_t)pt * a.n_coils + cc] = make_float2((float)(re * inv), (float)(im * inv));
   }
 }
@@ -839,7 +859,7 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   // fftshift folded in): the padded point (x, y, z) reads grid element
   // gix[0][x] + gix[1][y] + gix[2][z]
   __shared__ int gix[3][256];
-  for (int i = tid; i < g.K; i += NT) lut[i] = a.lut[i];
+  copy_to_lds<NT, 4>(lut, a.lut, g.K);
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
   for (int e = tid; e < DIM * P; e += NT) {
     const int d = e >= 2 * P ? 2 : (e >= P ? 1 : 0);
@@ -851,22 +871,30 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   if (DIM == 2 && tid == 0) gix[2][0] = 0;
   __syncthreads();
   // Stage the padded sector, element e = pt*CB + coil (the coil of thread
-  // tid is tid % CB in every iteration); zero the overrun slack.
+  // tid is tid % CB in every iteration); zero the overrun slack. Four
+  // loads per thread in flight before their LDS writes.
   {
+    constexpr int U = 4;
     const int cc = tid % CB;
     const bool coil_in = coil0 + cc < a.n_coils;
     const float2* gbase = a.grid + (size_t)(coil_in ? coil0 + cc : 0) * g.Ng;
-    for (int e = tid; e < L.sub_n; e += NT) {
-      const int pt = e / CB;
-      float2 v = make_float2(0.0f, 0.0f);
-      if (pt < PD && coil_in) {
-        const int yz = (int)udiv((uint32_t)pt, g.mP);
-        const int x = pt - yz * P;
-        const int z = DIM == 3 ? (int)udiv((uint32_t)pt, g.mP2) : 0;
-        const int y = yz - z * P;
-        v = gbase[(uint32_t)(gix[0][x] + gix[1][y] + gix[2][z])];
+    for (int e0 = tid; e0 < L.sub_n; e0 += U * NT) {
+      float2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int pt = (e0 + u * NT) / CB;
+        v[u] = make_float2(0.0f, 0.0f);
+        if (pt < PD && coil_in) {
+          const int yz = (int)udiv((uint32_t)pt, g.mP);
+          const int x = pt - yz * P;
+          const int z = DIM == 3 ? (int)udiv((uint32_t)pt, g.mP2) : 0;
+          const int y = yz - z * P;
+          v[u] = gbase[(uint32_t)(gix[0][x] + gix[1][y] + gix[2][z])];
+        }
       }
-      sub[e] = v;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (e0 + u * NT < L.sub_n) sub[e0 + u * NT] = v[u];
     }
   }
 
