@@ -1161,9 +1161,12 @@ hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStrea
 // ---------------------------------------------------------------------------
 // Batched launches: one launch per stage for a batch of operators
 // ---------------------------------------------------------------------------
+// Batch rounds are sized for four workgroups per CU (eight waves per SIMD);
+// the 2-D 4-tap spread fits that in 64 VGPRs without spilling, so it is
+// compiled for it (unconstrained it takes 70 and three workgroups fit).
 template <int DIM, int NX>
-__global__ void __launch_bounds__(kThreads) spread_batch_kernel(const BatchOp* __restrict__ ops,
-                                                                BatchList l) {
+__global__ void __launch_bounds__(kThreads, DIM == 2 && NX == 4 ? 8 : 1)
+    spread_batch_kernel(const BatchOp* __restrict__ ops, BatchList l) {
   uint4 it;
   if (!batch_item(l, blockIdx.x, &it)) return;
   const BatchOp& op = ops[it.x];
