@@ -44,6 +44,7 @@ constexpr int kBigTileElems = kPerThread * kBigThreads;
 constexpr int kMaxStripe = 32;    // lines per tile of a y / z pass (256 B rows)
 constexpr int kMaxPix = 4;        // image pixels per thread in the x passes
 constexpr int kMaxLen = kMaxPix * kFftThreads;  // longest transform (2048)
+constexpr int kCoilLoads = 4;  // per-pixel coil loads in flight in the x passes
 
 // ---------------------------------------------------------------------------
 // Small DFTs with kernel exp(sign * 2 pi i / R)
@@ -485,13 +486,23 @@ __device__ __forceinline__ void fft_adjoint_last_body(const LastPass& p, int blo
       const float d = p.apply_deapo ? p.deapo[pix] : 1.0f;
       if (p.sens) {
         float2 acc = add ? p.out[pix] : make_float2(0.f, 0.f);
-        for (int c = 0; c < ng; c++) {
-          float2 v = buf[(c * nq + q) * p.LP + m];
-          v.x = (v.x * p.scale) * d;
-          v.y = (v.y * p.scale) * d;
-          const float2 s = p.sens[(long long)(c0 + c) * Ni + pix];
-          acc.x += v.x * s.x + v.y * s.y;
-          acc.y += v.y * s.x - v.x * s.y;
+        // coils in groups of kCoilLoads: their sensitivity loads are in
+        // flight together (the sum keeps the coil order)
+        for (int cb = 0; cb < ng; cb += kCoilLoads) {
+          float2 sc[kCoilLoads];
+#pragma unroll
+          for (int u = 0; u < kCoilLoads; u++)
+            sc[u] = cb + u < ng ? p.sens[(long long)(c0 + cb + u) * Ni + pix] : make_float2(0.f, 0.f);
+#pragma unroll
+          for (int u = 0; u < kCoilLoads; u++) {
+            if (cb + u >= ng) break;
+            float2 v = buf[((cb + u) * nq + q) * p.LP + m];
+            v.x = (v.x * p.scale) * d;
+            v.y = (v.y * p.scale) * d;
+            const float2 s = sc[u];
+            acc.x += v.x * s.x + v.y * s.y;
+            acc.y += v.y * s.x - v.x * s.y;
+          }
         }
         p.out[pix] = acc;
       } else {
@@ -559,10 +570,18 @@ __device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int b
       const float d = p.deapo[pix];
       const int pos = wrap1(x + p.crop0 - p.shift0, p.G0);
       const float2 shared_img = p.img_per_coil ? make_float2(0.f, 0.f) : p.img[pix];
-      for (int c = 0; c < ng; c++) {
-        float2 v = p.img_per_coil ? p.img[(long long)(c0 + c) * Ni + pix] : shared_img;
-        if (p.sens) v = cmul(v, p.sens[(long long)(c0 + c) * Ni + pix]);
-        buf[(c * nq + q) * p.LP + pos] = make_float2(v.x * d, v.y * d);
+      for (int cb = 0; cb < ng; cb += kCoilLoads) {
+        // kCoilLoads coils' loads in flight before their LDS writes
+        float2 v[kCoilLoads];
+#pragma unroll
+        for (int u = 0; u < kCoilLoads; u++) {
+          const long long o = (long long)(c0 + cb + u) * Ni + pix;
+          v[u] = cb + u < ng && p.img_per_coil ? p.img[o] : shared_img;
+          if (cb + u < ng && p.sens) v[u] = cmul(v[u], p.sens[o]);
+        }
+#pragma unroll
+        for (int u = 0; u < kCoilLoads; u++)
+          if (cb + u < ng) buf[((cb + u) * nq + q) * p.LP + pos] = make_float2(v[u].x * d, v[u].y * d);
       }
     }
     __syncthreads();
