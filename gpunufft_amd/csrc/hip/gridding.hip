@@ -859,6 +859,9 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   // fftshift folded in): the padded point (x, y, z) reads grid element
   // gix[0][x] + gix[1][y] + gix[2][z]
   __shared__ int gix[3][256];
+  // round 0's coordinates first: their loads overlap the table set-up
+  float p0[3] = {0.0f, 0.0f, 0.0f};
+  if (tid < min(R, n)) load_coord<DIM>(a.crds, g.M, ch.begin + tid, p0);
   copy_to_lds<NT, 4>(lut, a.lut, g.K);
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
   for (int e = tid; e < DIM * P; e += NT) {
@@ -870,6 +873,7 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   }
   if (DIM == 2 && tid == 0) gix[2][0] = 0;
   __syncthreads();
+  if (tid < min(R, n)) make_record<DIM, NX>(g, lut, kp, KPS, c3, p0, rec + tid * RW);
   // Stage the padded sector, element e = pt*CB + coil (the coil of thread
   // tid is tid % CB in every iteration); zero the overrun slack. Four
   // loads per thread in flight before their LDS writes.
@@ -906,13 +910,15 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
 
   for (int r0 = 0; r0 < n; r0 += R) {
     const int rn = min(R, n - r0);
-    __syncthreads();  // tables ready / previous round consumed
-    for (int i = tid; i < rn; i += NT) {
-      float p[3];
-      load_coord<DIM>(a.crds, g.M, ch.begin + r0 + i, p);
-      make_record<DIM, NX>(g, lut, kp, KPS, c3, p, rec + i * RW);
+    if (r0 > 0) {
+      __syncthreads();  // previous round consumed
+      for (int i = tid; i < rn; i += NT) {
+        float p[3];
+        load_coord<DIM>(a.crds, g.M, ch.begin + r0 + i, p);
+        make_record<DIM, NX>(g, lut, kp, KPS, c3, p, rec + i * RW);
+      }
     }
-    __syncthreads();
+    __syncthreads();  // sub-grid staged, the round's records ready
 
     // consecutive samples per wave step: the scattered stores of one wave
     // cover runs of neighbouring k-space samples
