@@ -49,9 +49,9 @@ def parse():
     ap.add_argument("--requests", type=int, default=40, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=30.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent requests")
-    ap.add_argument("--sector-widths", default="8",
+    ap.add_argument("--sector-widths", default="4,8,16",
                     help="uneven_batch_2d: comma-separated sector widths the autotuner "
-                         "chooses from per operator (e.g. 4,8,16)")
+                         "chooses from per operator (one width: no tuning)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps (for rocprofv3), no baseline")
     ap.add_argument("--profile-copy-mb", type=int, default=0,

@@ -9,6 +9,13 @@ data with HIP events, and returns the fastest width. Results are cached per
 (image size, kernel width, osf, sample count, coil count) so operators of one
 shape are tuned once.
 
+A batch runs every operator inside launches that fill the GPU, so what an
+operator costs there is its share of a full GPU, not the latency of a lone
+launch (which favours small sectors: more workgroups for a small operator).
+With fill_blocks set (the default), the timed launches repeat the coil set
+until they hold about that many workgroups, and the time is divided by the
+repeat count.
+
 Workgroup size and coil blocking follow from the width: the kernels size
 their LDS rounds and coil blocks to the padded sector of the chosen width.
 """
@@ -21,17 +28,18 @@ _CACHE: dict = {}
 
 
 def best_sector_width(img_dims, kernel_width, osf, traj, dens, n_coils, widths=(4, 8, 16),
-                      lookup=capi.LOOKUP_NEAREST, reps=3):
+                      lookup=capi.LOOKUP_NEAREST, reps=3, fill_blocks=4096, max_repeat=16):
     """traj (d, M) / dens (M,) float32 device tensors; returns the width of
-    `widths` with the shortest spread + interp time."""
+    `widths` with the shortest spread + interp time per coil set, timed with
+    the coil set repeated (at most max_repeat times) until a launch holds
+    about fill_blocks workgroups (fill_blocks=0: the operator alone)."""
     import torch
 
     key = (tuple(img_dims), kernel_width, float(osf), int(traj.shape[1]), int(n_coils),
-           tuple(widths), lookup)
+           tuple(widths), lookup, int(fill_blocks), int(max_repeat))
     if key in _CACHE:
         return _CACHE[key]
     M = traj.shape[1]
-    data = torch.zeros((n_coils, M), dtype=torch.complex64, device="cuda")
     best, best_t = None, None
     for sw in widths:
         try:
@@ -39,23 +47,29 @@ def best_sector_width(img_dims, kernel_width, osf, traj, dens, n_coils, widths=(
         except capi.GnufftError:
             continue  # padded sector too large for this width
         p.set_trajectory(traj, dens)
-        g = p.info().grid
-        gshape = (n_coils,) + tuple(int(x) for x in g[:len(img_dims)][::-1])
+        info = p.info()
+        rep = 1
+        if fill_blocks:
+            per_set = max(1, info.n_chunks) * -(-n_coils // 8)  # chunks x 8-coil blocks
+            rep = max(1, min(max_repeat, -(-fill_blocks // per_set)))
+        C = n_coils * rep
+        data = torch.zeros((C, M), dtype=torch.complex64, device="cuda")
+        gshape = (C,) + tuple(int(x) for x in info.grid[:len(img_dims)][::-1])
         grid = torch.empty(gshape, dtype=torch.complex64, device="cuda")
-        out = torch.empty((n_coils, M), dtype=torch.complex64, device="cuda")
-        p.spread_sorted(data, grid, n_coils)
-        p.interp_sorted(grid, out, n_coils)
+        out = torch.empty((C, M), dtype=torch.complex64, device="cuda")
+        p.spread_sorted(data, grid, C)
+        p.interp_sorted(grid, out, C)
         ts = []
         for _ in range(reps):
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record()
-            p.spread_sorted(data, grid, n_coils)
-            p.interp_sorted(grid, out, n_coils)
+            p.spread_sorted(data, grid, C)
+            p.interp_sorted(grid, out, C)
             b.record()
             b.synchronize()
             ts.append(a.elapsed_time(b))
-        t = min(ts)
+        t = min(ts) / rep
         if best_t is None or t < best_t:
             best, best_t = sw, t
         p.close()

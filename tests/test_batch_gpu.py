@@ -106,3 +106,19 @@ def test_batch_rejects_mixed_dimensions(env):
     p3.set_trajectory(_dev(torch, W.koosh_3d(8, 8)))
     with pytest.raises(capi.GnufftError):
         capi.Batch([p2, p3], [1, 1])
+
+
+def test_autotuner_picks_a_candidate_width():
+    """The sector-width autotuner returns one of its candidates, for the
+    operator alone and at full-GPU occupancy, and caches per shape."""
+    import torch
+    from gpunufft_amd import autotune
+    t = W.spiral_2d(8, 1024, n_turns=8)
+    traj = torch.from_numpy(t).cuda()
+    dens = torch.from_numpy(W.ramp_density(t)).cuda()
+    for fill in (0, 4096):
+        sw = autotune.best_sector_width((96, 96), 3, 2.0, traj, dens, 4, widths=(8, 16),
+                                        fill_blocks=fill)
+        assert sw in (8, 16)
+        assert autotune.best_sector_width((96, 96), 3, 2.0, traj, dens, 4, widths=(8, 16),
+                                          fill_blocks=fill) == sw
