@@ -707,10 +707,13 @@ hipError_t allow_lds(K kernel, size_t lds) {
 // x-pass tile: Cg coils x Q lines with Cg * Q * LP <= kTileElems and
 // Q * N0 <= kMaxPix * kFftThreads (N0 <= G0 <= kMaxLen keeps Q >= 1 inside
 // both bounds); a multiple of 32 lines where that fits (one line per lane).
+// Coils that need several groups are split into equal groups (16 coils at
+// 15 lines per tile: 8 + 8, not 15 + 1).
 void x_tiling(int G0, int N0, int coils, int* Cg, int* Q) {
   int per_tile = std::max(1, kTileElems / pad_of(G0));
   if (per_tile >= 32) per_tile -= per_tile % 32;
-  *Cg = std::max(1, std::min(coils, per_tile));
+  const int groups = (std::max(1, coils) + per_tile - 1) / per_tile;
+  *Cg = std::max(1, (coils + groups - 1) / groups);
   *Q = std::max(1, std::min(per_tile / *Cg, (kMaxPix * kFftThreads) / N0));
 }
 

@@ -145,3 +145,29 @@ def test_fused_headline_geometry_matches_rocfft(env):
 
     assert rel(outs[capi.FFT_FUSED][0], outs[capi.FFT_ROCFFT][0]) < TOL
     assert rel(outs[capi.FFT_FUSED][1], outs[capi.FFT_ROCFFT][1]) < TOL
+
+
+def test_fused_many_coils_long_lines(oracle, env):
+    """16 coils on 512-point lines: the x passes split the coils into equal
+    groups (15 lines of 513 words fill a tile); adjoint with the coil
+    combine accumulating across groups, and forward."""
+    torch, capi = env
+    dims, osf, C = (256, 256), 2.0, 16
+    traj = _traj(dims, 3)
+    dens = W.ramp_density(traj, 1)
+    sens = W.gaussian_sens(dims, C, seed=5)
+    rng = np.random.default_rng(9)
+    kd = W.complex_normal(rng, (C, traj.shape[1]))
+    img = W.complex_normal(rng, (1,) + tuple(dims[::-1]))
+    op = oracle.Operator(traj, dims, 3, 8, osf, dens=dens, sens=sens, order="stable")
+    p = capi.Plan(dims, 3, 8, osf)
+    p.set_trajectory(_dev(torch, traj), _dev(torch, dens))
+    p.set_sens(_dev(torch, sens), C)
+    assert p.fft() == capi.FFT_FUSED
+    a = torch.empty((1,) + tuple(dims[::-1]), dtype=torch.complex64, device="cuda")
+    f = torch.empty((C, traj.shape[1]), dtype=torch.complex64, device="cuda")
+    p.adjoint(_dev(torch, kd), a, C)
+    p.forward(_dev(torch, img), f, C)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(a.cpu().numpy(), op.adjoint(kd)) < TOL
+    assert oracle.rel_l2(f.cpu().numpy(), op.forward(img)) < TOL
