@@ -551,7 +551,6 @@ SpreadArgs spread_args(gnufft_plan p, const float2* data, float2* grid, uint32_t
   a.coil_stride = p->M;
   a.crds = p->traj_sorted_d;
   a.work = p->work;
-  a.centers = p->centers_d;
   a.lut = p->lut_d;
   a.scratch = p->scratch_d;
   a.sector_chunks = p->sector_chunks_d;
@@ -994,7 +993,6 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   a.grid = (const float2*)grid;
   a.crds = p->traj_sorted_d;
   a.work = p->work;
-  a.centers = p->centers_d;
   a.lut = p->lut_d;
   a.out = (float2*)data_sorted;
   a.coil_stride = p->M;
@@ -1084,7 +1082,6 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.grid = p->img_tmp_d;
     a.crds = p->traj_sorted_d;
     a.work = p->work;
-    a.centers = p->centers_d;
     a.lut = p->lut_d;
     a.out = (float2*)out;
     a.coil_stride = p->M;
@@ -1119,7 +1116,6 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.grid = p->grid_d;
     a.crds = p->traj_sorted_d;
     a.work = p->work;
-    a.centers = p->centers_d;
     a.lut = p->lut_d;
     a.out = (float2*)out + (size_t)c0 * p->M;
     a.coil_stride = p->M;
@@ -1305,7 +1301,6 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     op.i.grid = b->grid_d + grid_off[k];
     op.i.crds = p->traj_sorted_d;
     op.i.work = p->work;
-    op.i.centers = p->centers_d;
     op.i.lut = p->lut_d;
     op.i.coil_stride = p->M;
     op.i.scatter = p->idx_d;

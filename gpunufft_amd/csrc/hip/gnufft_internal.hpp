@@ -209,7 +209,6 @@ struct SpreadArgs {
   const float* dens;       // NULL: no density weighting (sorted order)
   const float* crds;       // sorted SoA [dim][M]
   WorkList work;
-  const uint32_t* centers; // [S][dim]
   const float* lut;
   float2* scratch;         // [chunk][padded point][coil] sub-grids (n_chunks*PD*n_coils)
   const uint32_t* sector_chunks;  // [S+1] first chunk of each sector
@@ -226,7 +225,6 @@ struct InterpArgs {
   const float2* grid;      // [coil][Ng]
   const float* crds;
   WorkList work;
-  const uint32_t* centers;
   const float* lut;
   float2* out;             // [coil][...]
   long long coil_stride;

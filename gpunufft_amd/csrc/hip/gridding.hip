@@ -1133,10 +1133,8 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStrea
   }
   // the fold writes every grid point, zeros where no sample contributes
   const uint32_t S = (uint32_t)g.S[0] * g.S[1] * g.S[2];
-  const int hp = g.dim == 3 ? g.sw * g.sw * g.sw : g.sw * g.sw;
   const int threads = fold_threads(g, a.cover);
   if (threads == 0) return hipErrorInvalidValue;
-  (void)hp;
   const int cb = a.n_coils >= 8 ? 8 : a.n_coils >= 4 ? 4 : a.n_coils >= 2 ? 2 : 1;
   const dim3 grid(S, (a.n_coils + cb - 1) / cb);
   if (g.dim == 3) {
