@@ -1241,13 +1241,13 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     return ga.G[0] < gc.G[0];
   });
   // One launch per stage uses the largest footprint of any plan, so each
-  // plan's round size is the largest that still lets four workgroups share
-  // a CU (then three, two, one): one dense plan must not cut the occupancy
-  // of the whole batch.
+  // plan's round size is the largest that still lets kBatchPerCU workgroups
+  // share a CU (then fewer): one dense plan must not cut the occupancy of
+  // the whole batch.
   auto batch_round = [](int (*round)(const Geom&, int, int, size_t), const Geom& g, int target,
                         size_t reserve) {
     const int need = std::min(64, target);
-    for (int k = 4; k >= 1; k--) {
+    for (int k = kBatchPerCU; k >= 1; k--) {
       const int r = round(g, kBatchCB, target, lds_budget() / k - 1024 - reserve);
       if (r >= need || (k == 1 && r > 0)) return r;
     }

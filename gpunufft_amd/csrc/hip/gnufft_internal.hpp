@@ -292,8 +292,14 @@ struct BatchList {
 // Split items of the given costs into the 8 XCD ranges of `l` (items stay in order).
 void partition_items(const uint32_t* costs, uint32_t n, BatchList* l);
 
-// gridding.hip (CB: coils per workgroup, the same for every operator)
+// gridding.hip (CB: coils per workgroup, the same for every operator).
+// Batched spread / interp workgroups have kBatchThreads threads and their
+// rounds are sized so that kBatchPerCU of them share a CU. (Smaller
+// workgroups do not raise that count: the 8-coil sub-grid dominates the LDS
+// footprint, 20.7 KB at 2-D sw 16.)
 constexpr int kBatchCB = 8;
+constexpr int kBatchThreads = 512;
+constexpr int kBatchPerCU = 4;
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
 hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,

@@ -284,10 +284,9 @@ __device__ inline void copy_to_lds(float* dst, const float* __restrict__ src, in
 
 // One workgroup's share of the spread: chunk ci of the work list, coils
 // [coil_block * CB, coil_block * CB + CB).
-template <int DIM, int NX, int CB>
+template <int DIM, int NX, int CB, int NT = kThreads>
 __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t ci, int coil_block,
                                    int R) {
-  constexpr int NT = kThreads;
   constexpr int NW = NT / 64;
   constexpr int SPW = 64 / CB;  // sample slots per wave; one lane per (slot, coil)
   constexpr int RW = Rec<NX>::W;
@@ -828,10 +827,9 @@ void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], in
 // ---------------------------------------------------------------------------
 // Interp
 // ---------------------------------------------------------------------------
-template <int DIM, int NX, int CB>
+template <int DIM, int NX, int CB, int NT = kThreads>
 __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t ci, int coil_block,
                                    int R) {
-  constexpr int NT = kThreads;
   constexpr int NW = NT / 64;
   constexpr int SPW = 64 / CB;  // sample slots per wave; one lane per (slot, coil)
   constexpr int RW = Rec<NX>::W;
@@ -1008,15 +1006,16 @@ static hipError_t allow_lds(K kernel, size_t lds) {
 static int nx_of(const Geom& g) { return g.kw + 1 <= 4 ? 4 : 8; }
 
 // Samples per round: as many as fit next to the sub-grid (budget `limit`),
-// at most one per thread and at most the chunk; multiple of 16.
+// at most one per thread of a `threads`-thread workgroup and at most the
+// chunk; multiple of 16.
 template <typename Layout>
-static int round_size(const Geom& g, int cb, int chunk, size_t limit) {
+static int round_size(const Geom& g, int cb, int chunk, size_t limit, int threads = kThreads) {
   const int nx = nx_of(g);
   const size_t fixed = Layout(g, nx, cb, 0).bytes;
   const size_t per = Layout(g, nx, cb, 1).bytes - fixed;
   if (fixed + 16 * per > limit) return 0;
   size_t r = (limit - fixed) / per;
-  r = r > (size_t)kThreads ? (size_t)kThreads : r;
+  r = r > (size_t)threads ? (size_t)threads : r;
   r &= ~(size_t)15;
   const int want = (chunk + 15) & ~15;
   return (int)r < want ? (int)r : want;
@@ -1165,25 +1164,25 @@ hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStrea
 // ---------------------------------------------------------------------------
 // Batched launches: one launch per stage for a batch of operators
 // ---------------------------------------------------------------------------
-// Batch rounds are sized for four workgroups per CU (eight waves per SIMD);
-// the 2-D 4-tap spread fits that in 64 VGPRs without spilling, so it is
-// compiled for it (unconstrained it takes 70 and three workgroups fit).
+// Batch rounds are sized for kBatchPerCU workgroups of kBatchThreads per
+// CU (eight waves per SIMD); the 2-D 4-tap spread fits that in 64 VGPRs
+// without spilling, so it is compiled for it (unconstrained it takes 70).
 template <int DIM, int NX>
-__global__ void __launch_bounds__(kThreads, DIM == 2 && NX == 4 ? 8 : 1)
+__global__ void __launch_bounds__(kBatchThreads, DIM == 2 && NX == 4 ? 8 : 1)
     spread_batch_kernel(const BatchOp* __restrict__ ops, BatchList l) {
   uint4 it;
   if (!batch_item(l, blockIdx.x, &it)) return;
   const BatchOp& op = ops[it.x];
-  spread_body<DIM, NX, kBatchCB>(op.g, op.s, it.y, (int)it.z, op.R_spread);
+  spread_body<DIM, NX, kBatchCB, kBatchThreads>(op.g, op.s, it.y, (int)it.z, op.R_spread);
 }
 
 template <int DIM, int NX>
-__global__ void __launch_bounds__(kThreads) interp_batch_kernel(const BatchOp* __restrict__ ops,
-                                                                BatchList l) {
+__global__ void __launch_bounds__(kBatchThreads) interp_batch_kernel(const BatchOp* __restrict__ ops,
+                                                                     BatchList l) {
   uint4 it;
   if (!batch_item(l, blockIdx.x, &it)) return;
   const BatchOp& op = ops[it.x];
-  interp_body<DIM, NX, kBatchCB>(op.g, op.i, it.y, (int)it.z, op.R_interp);
+  interp_body<DIM, NX, kBatchCB, kBatchThreads>(op.g, op.i, it.y, (int)it.z, op.R_interp);
 }
 
 template <int DIM>
@@ -1216,11 +1215,11 @@ size_t interp_lds_bytes(const Geom& g, int cb, int R) {
 }
 
 int spread_round(const Geom& g, int cb, int chunk, size_t limit) {
-  return round_size<SpreadLayout>(g, cb, chunk, limit);
+  return round_size<SpreadLayout>(g, cb, chunk, limit, kBatchThreads);
 }
 
 int interp_round(const Geom& g, int cb, int chunk, size_t limit) {
-  return round_size<InterpLayout>(g, cb, chunk, limit);
+  return round_size<InterpLayout>(g, cb, chunk, limit, kBatchThreads);
 }
 
 template <typename K>
@@ -1236,19 +1235,19 @@ static hipError_t launch_items(K kernel, const BatchOp* ops, const BatchList& l,
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s) {
   if (dim == 3)
-    return nx == 4 ? launch_items(spread_batch_kernel<3, 4>, ops, l, kThreads, lds, s)
-                   : launch_items(spread_batch_kernel<3, 8>, ops, l, kThreads, lds, s);
-  return nx == 4 ? launch_items(spread_batch_kernel<2, 4>, ops, l, kThreads, lds, s)
-                 : launch_items(spread_batch_kernel<2, 8>, ops, l, kThreads, lds, s);
+    return nx == 4 ? launch_items(spread_batch_kernel<3, 4>, ops, l, kBatchThreads, lds, s)
+                   : launch_items(spread_batch_kernel<3, 8>, ops, l, kBatchThreads, lds, s);
+  return nx == 4 ? launch_items(spread_batch_kernel<2, 4>, ops, l, kBatchThreads, lds, s)
+                 : launch_items(spread_batch_kernel<2, 8>, ops, l, kBatchThreads, lds, s);
 }
 
 hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s) {
   if (dim == 3)
-    return nx == 4 ? launch_items(interp_batch_kernel<3, 4>, ops, l, kThreads, lds, s)
-                   : launch_items(interp_batch_kernel<3, 8>, ops, l, kThreads, lds, s);
-  return nx == 4 ? launch_items(interp_batch_kernel<2, 4>, ops, l, kThreads, lds, s)
-                 : launch_items(interp_batch_kernel<2, 8>, ops, l, kThreads, lds, s);
+    return nx == 4 ? launch_items(interp_batch_kernel<3, 4>, ops, l, kBatchThreads, lds, s)
+                   : launch_items(interp_batch_kernel<3, 8>, ops, l, kBatchThreads, lds, s);
+  return nx == 4 ? launch_items(interp_batch_kernel<2, 4>, ops, l, kBatchThreads, lds, s)
+                 : launch_items(interp_batch_kernel<2, 8>, ops, l, kBatchThreads, lds, s);
 }
 
 hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, int threads,
