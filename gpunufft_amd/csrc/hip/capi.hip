@@ -740,7 +740,8 @@ int gnufft_plan_set_fft(gnufft_plan p, int kind) {
   if (kind != GNUFFT_FFT_FUSED) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_fft: unknown kind");
   if (!p->fused_ok)
     return fail(GNUFFT_ERR_UNSUPPORTED,
-                "set_fft: fused FFT needs grid lengths 2^a 3^b 5^c up to 2048");
+                "set_fft: fused FFT needs grid lengths with prime factors 2, 3, 5, 7, 11, 13, 17, 19 "
+                "up to 2048");
   p->fused = true;
   return GNUFFT_OK;
 }
@@ -1227,10 +1228,14 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
   }
   gnufft_batch b = new (std::nothrow) gnufft_batch_s();
   if (!b) return fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: host allocation");
+  // plans on the fused FFT run it inside the batch unless a line length
+  // needs the prime-radix stages (the batch kernels omit them): those take
+  // rocFFT and the separate pre / post kernels here
+  auto batch_fused = [](const gnufft_plan q) { return q->fused && fused_fft_batchable(q->g); };
   b->dim = g0.dim;
   b->nx = nx;
   b->fused = false;
-  for (uint32_t i = 0; i < n_plans; i++) b->fused = b->fused || plans[i]->fused;
+  for (uint32_t i = 0; i < n_plans; i++) b->fused = b->fused || batch_fused(plans[i]);
   // batch order: grouped by grid size (one rocFFT plan per group)
   std::vector<uint32_t> order(n_plans);
   for (uint32_t i = 0; i < n_plans; i++) order[i] = i;
@@ -1261,17 +1266,17 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     b->plans.push_back(p);
     b->coils.push_back(C);
     b->caller_index.push_back(order[k]);
-    b->op_fused.push_back(p->fused ? 1 : 0);
+    b->op_fused.push_back(batch_fused(p) ? 1 : 0);
     grid_off[k] = grid_total;
     grid_total += (size_t)C * p->g.Ng;
     scratch_off[k] = scratch_total;
     // spread sub-grids, then (fused FFT) the transform's intermediate array
     scratch_total += std::max((size_t)p->n_chunks * C * padded_points(p->g),
-                              p->fused ? (size_t)C * fused_fft_tmp_elems(p->g) : (size_t)0);
+                              batch_fused(p) ? (size_t)C * fused_fft_tmp_elems(p->g) : (size_t)0);
     // rocFFT groups: runs of consecutive non-fused plans of one grid size
-    if (!p->fused) {
+    if (!batch_fused(p)) {
       const gnufft_plan q = k ? plans[order[k - 1]] : nullptr;
-      if (!q || q->fused || p->g.G[0] != q->g.G[0] || p->g.G[1] != q->g.G[1] ||
+      if (!q || batch_fused(q) || p->g.G[0] != q->g.G[0] || p->g.G[1] != q->g.G[1] ||
           p->g.G[2] != q->g.G[2])
         b->groups.push_back({k, 0, grid_off[k], FftPlans()});
       b->groups.back().count++;
@@ -1329,7 +1334,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     b->max_kstride = std::max<uint32_t>(b->max_kstride, C * (uint32_t)padded_points(p->g));
     b->ops_h.push_back(op);
     if (b->fused) b->fft_desc_h.resize((k + 1) * fused_fft_desc_bytes());
-    if (p->fused) {
+    if (batch_fused(p)) {
       const size_t db = fused_fft_desc_bytes();
       if (!fused_fft_desc_build(b->fft_desc_h.data() + k * db, p->g, b->grid_d + grid_off[k],
                                 b->scratch_d + scratch_off[k], (int)C, p->deapo_d,
@@ -1361,7 +1366,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
         fo.push_back(make_uint4(k, sct, cb, 0));
         fo_cost.push_back(1u);
       }
-    if (!p->fused) {  // pre / post run beside rocFFT; the fused FFT has them inside
+    if (!batch_fused(p)) {  // pre / post run beside rocFFT; the fused FFT has them inside
       for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ni; blk++) po.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
       for (uint64_t blk = 0; blk * 256 < (uint64_t)p->g.Ng; blk++) pr.push_back(make_uint4(k, (uint32_t)blk, 0, 0));
     }

@@ -129,6 +129,58 @@ __device__ __forceinline__ void dft<5>(float2* v, float sign) {
   v[3] = csub(m2, n2);
 }
 
+// Odd prime R (7, 11, 13, 17, 19): the direct DFT, inputs paired as
+// x_n +- x_{R-n} so that X_k and X_{R-k} share their sums:
+// X_k = x_0 + sum_n (x_n + x_{R-n}) cos(2 pi n k / R)
+//           + i sum_n (x_n - x_{R-n}) sign sin(2 pi n k / R), n = 1 .. (R-1)/2.
+// The constants come from the stage's twiddle table: w[m * step] =
+// exp(sign 2 pi i m / R) for a line of step * R points.
+template <int R>
+__device__ __forceinline__ void dft_prime(float2* v, const float2* w, int step) {
+  constexpr int H = (R - 1) / 2;
+  float c[H], sn[H];
+#pragma unroll
+  for (int m = 1; m <= H; m++) {
+    const float2 e = w[m * step];
+    c[m - 1] = e.x;
+    sn[m - 1] = e.y;
+  }
+  float2 t[H], u[H];
+  const float2 x0 = v[0];
+  float2 sum = x0;
+#pragma unroll
+  for (int n = 1; n <= H; n++) {
+    t[n - 1] = cadd(v[n], v[R - n]);
+    u[n - 1] = csub(v[n], v[R - n]);
+    sum = cadd(sum, t[n - 1]);
+  }
+  v[0] = sum;
+#pragma unroll
+  for (int k = 1; k <= H; k++) {
+    float2 a = x0, b = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int n = 1; n <= H; n++) {
+      const int m = (n * k) % R;  // compile-time
+      const float cm = m <= H ? c[m - 1] : c[R - m - 1];
+      const float sm = m <= H ? sn[m - 1] : -sn[R - m - 1];
+      a = make_float2(a.x + t[n - 1].x * cm, a.y + t[n - 1].y * cm);
+      b = make_float2(b.x + u[n - 1].x * sm, b.y + u[n - 1].y * sm);
+    }
+    v[k] = make_float2(a.x - b.y, a.y + b.x);      // a + i b
+    v[R - k] = make_float2(a.x + b.y, a.y - b.x);  // a - i b
+  }
+}
+
+// The DFT of one stage: closed forms for 2, 3, 4, 5, 8; the direct form for
+// the larger primes (constants from the twiddles w of the line).
+template <int R>
+__device__ __forceinline__ void dft_stage(float2* v, float sign, const float2* w, int step) {
+  if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
+    dft<R>(v, sign);
+  else
+    dft_prime<R>(v, w, step);
+}
+
 // n / d for n * d < 2^32 via a multiply-high with m = ceil(2^32 / d); d == 1
 // has no 32-bit m and is passed as m == 0.
 __device__ __forceinline__ int qdiv(int n, uint32_t m) {
@@ -182,7 +234,7 @@ __device__ __forceinline__ void finish_butterfly(float2* v, int j, int l, const 
 #pragma unroll
     for (int s = 1; s < R; s++) v[s] = cmul(v[s], tw[k * s * st.tws]);
   }
-  dft<R>(v, sign);
+  dft_stage<R>(v, sign, tw, st.per_line);
   const int base = jq * st.Ns * R + k;
 #pragma unroll
   for (int s = 0; s < R; s++) put(l, base + s * st.Ns, v[s]);
@@ -219,7 +271,9 @@ __device__ __forceinline__ void stage(int lines, uint32_t m_lines, const FftStag
   }
 }
 
-template <int NT = kFftThreads, typename Get, typename Put>
+// PR: also the prime radices 7 .. 19 (their registers would otherwise set
+// the occupancy of every table-driven pass).
+template <int NT = kFftThreads, bool PR = false, typename Get, typename Put>
 __device__ __forceinline__ void stage_any(int lines, uint32_t m_lines, const FftStage& table_entry,
                                  const float2* tw, float sign, bool sync, Get get, Put put) {
   // a register copy: the table lives in global memory, and stores of the
@@ -230,18 +284,30 @@ __device__ __forceinline__ void stage_any(int lines, uint32_t m_lines, const Fft
     case 4: stage<4, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
     case 2: stage<2, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
     case 3: stage<3, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
-    default: stage<5, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    case 5: stage<5, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+    default:
+      if constexpr (PR) {
+        switch (st.R) {
+          case 7: stage<7, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+          case 11: stage<11, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+          case 13: stage<13, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+          case 17: stage<17, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+          default: stage<19, NT>(lines, m_lines, st, tw, sign, sync, get, put); break;
+        }
+      }
+      break;
   }
 }
 
 // Transform `lines` lines held in LDS (stride LP) in place. Callers
 // synchronise after filling the lines; every stage ends with a barrier.
+template <bool PR>
 __device__ __forceinline__ void fft_lines(float2* buf, int lines, uint32_t m_lines, int LP,
                                  const FftStage* st, int n_st, const float2* tw, float sign) {
   auto get = [&](int l, int i) { return buf[l * LP + i]; };
   auto put = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
   for (int i = 0; i < n_st; i++) {
-    stage_any(lines, m_lines, st[i], tw, sign, true, get, put);
+    stage_any<kFftThreads, PR>(lines, m_lines, st[i], tw, sign, true, get, put);
     __syncthreads();
   }
 }
@@ -261,12 +327,13 @@ struct FixedStages {
 };
 
 // Stage runners: all stages of `lines` LDS lines (the x passes).
+template <bool PR>
 struct RunTable {
   const FftStage* st;
   int n;
   __device__ __forceinline__ void operator()(float2* buf, int lines, uint32_t m_lines, int LP,
                                              const float2* tw, float sign) const {
-    fft_lines(buf, lines, m_lines, LP, st, n, tw, sign);
+    fft_lines<PR>(buf, lines, m_lines, LP, st, n, tw, sign);
   }
 };
 
@@ -318,7 +385,7 @@ struct AxisPass {
   const float2* tw;
 };
 
-template <int NT>
+template <int NT, bool PR>
 __device__ __forceinline__ void fft_axis_body(const AxisPass& p, int block) {
   extern __shared__ float2 lds[];
   float2* tw = lds;
@@ -356,24 +423,26 @@ __device__ __forceinline__ void fft_axis_body(const AxisPass& p, int block) {
   const int n = p.n_st;
   __syncthreads();  // twiddles
   if (n == 1) {
-    stage_any<NT>(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_global);
+    stage_any<NT, PR>(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_global);
     return;
   }
-  stage_any<NT>(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_lds);
+  stage_any<NT, PR>(S, p.m_stripe, p.st[0], tw, p.sign, false, get_global, put_lds);
   __syncthreads();
   for (int i = 1; i < n - 1; i++) {
-    stage_any<NT>(S, p.m_stripe, p.st[i], tw, p.sign, true, get_lds, put_lds);
+    stage_any<NT, PR>(S, p.m_stripe, p.st[i], tw, p.sign, true, get_lds, put_lds);
     __syncthreads();
   }
-  stage_any<NT>(S, p.m_stripe, p.st[n - 1], tw, p.sign, false, get_lds, put_global);
+  stage_any<NT, PR>(S, p.m_stripe, p.st[n - 1], tw, p.sign, false, get_lds, put_global);
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_kernel(AxisPass p) {
-  fft_axis_body<kFftThreads>(p, blockIdx.x);
+  fft_axis_body<kFftThreads, PR>(p, blockIdx.x);
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kBigThreads, 1) fft_axis_big_kernel(AxisPass p) {
-  fft_axis_body<kBigThreads>(p, blockIdx.x);
+  fft_axis_body<kBigThreads, PR>(p, blockIdx.x);
 }
 
 // The same pass for a line length L = R0 R1 R2 and tile of S lines known at
@@ -598,8 +667,9 @@ __device__ __forceinline__ void fft_forward_first_body(const FirstPass& p, int b
   }
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPass p) {
-  fft_adjoint_last_body(p, blockIdx.x, RunTable{p.st, p.n_st});
+  fft_adjoint_last_body(p, blockIdx.x, RunTable<PR>{p.st, p.n_st});
 }
 
 template <int L, int R0, int R1, int R2>
@@ -607,8 +677,9 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_fixed_kernel(
   fft_adjoint_last_body(p, blockIdx.x, RunFixed<L, R0, R1, R2>{});
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(FirstPass p) {
-  fft_forward_first_body(p, blockIdx.x, RunTable{p.st, p.n_st});
+  fft_forward_first_body(p, blockIdx.x, RunTable<PR>{p.st, p.n_st});
 }
 
 template <int L, int R0, int R1, int R2>
@@ -637,7 +708,7 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_batch_kernel(const Ff
   // register copies: the descriptors live in global memory, which the
   // passes also store to
   const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
-  fft_axis_body<kFftThreads>(p, (int)it.y);
+  fft_axis_body<kFftThreads, false>(p, (int)it.y);
 }
 
 __global__ void __launch_bounds__(kBigThreads, 1) fft_axis_batch_big_kernel(
@@ -645,21 +716,21 @@ __global__ void __launch_bounds__(kBigThreads, 1) fft_axis_batch_big_kernel(
   const uint2 it = items[blockIdx.x];
   const FftOpDesc& o = d[it.x];
   const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
-  fft_axis_body<kBigThreads>(p, (int)it.y);
+  fft_axis_body<kBigThreads, false>(p, (int)it.y);
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_batch_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const LastPass p = d[it.x].adj_x;
-  fft_adjoint_last_body(p, (int)it.y, RunTable{p.st, p.n_st});
+  fft_adjoint_last_body(p, (int)it.y, RunTable<false>{p.st, p.n_st});
 }
 
 __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_batch_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const FirstPass p = d[it.x].fwd_x;
-  fft_forward_first_body(p, (int)it.y, RunTable{p.st, p.n_st});
+  fft_forward_first_body(p, (int)it.y, RunTable<false>{p.st, p.n_st});
 }
 
 // ---------------------------------------------------------------------------
@@ -670,7 +741,7 @@ uint32_t qdiv_const(int d) { return d == 1 ? 0u : udiv_magic((uint32_t)d); }
 bool make_radices(int L, FftRadices* rad) {
   rad->n = 0;
   int n = L;
-  const int order[5] = {8, 4, 2, 3, 5};
+  const int order[10] = {8, 4, 2, 3, 5, 7, 11, 13, 17, 19};
   for (int r : order)
     while (n > 1 && n % r == 0) {
       if (rad->n == 12) return false;
@@ -689,6 +760,10 @@ bool make_radices(int L, FftRadices* rad) {
   }
   return true;
 }
+
+// Lines whose transform has a stage of radix 7, 11, 13, 17 or 19 (run by
+// the kernel variants compiled with those stages).
+bool prime_radix(int L) { return L % 7 == 0 || L % 11 == 0 || L % 13 == 0 || L % 17 == 0 || L % 19 == 0; }
 
 // odd line pitch: the lanes of a wave, one line each, hit distinct banks
 int pad_of(int L) { return (L + 1) | 1; }
@@ -775,8 +850,12 @@ hipError_t launch_axis(const AxisPass& p, hipStream_t s) {
     return launch_pass(fft_axis_fixed_kernel<LEN, TILE, RA, RB, RC>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_AXIS_CASE)
 #undef GNUFFT_AXIS_CASE
-  if (p.big) return launch_pass(fft_axis_big_kernel, blocks, lds, s, p, kBigThreads);
-  return launch_pass(fft_axis_kernel, blocks, lds, s, p);
+  if (prime_radix(p.L)) {
+    if (p.big) return launch_pass(fft_axis_big_kernel<true>, blocks, lds, s, p, kBigThreads);
+    return launch_pass(fft_axis_kernel<true>, blocks, lds, s, p);
+  }
+  if (p.big) return launch_pass(fft_axis_big_kernel<false>, blocks, lds, s, p, kBigThreads);
+  return launch_pass(fft_axis_kernel<false>, blocks, lds, s, p);
 }
 
 template <typename K, typename P>
@@ -794,7 +873,8 @@ hipError_t launch_last(const LastPass& p, hipStream_t s) {
   if (p.G0 == LEN) return launch_x(fft_adjoint_last_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_LAST_CASE)
 #undef GNUFFT_LAST_CASE
-  return launch_x(fft_adjoint_last_kernel, blocks, lds, s, p);
+  if (prime_radix(p.G0)) return launch_x(fft_adjoint_last_kernel<true>, blocks, lds, s, p);
+  return launch_x(fft_adjoint_last_kernel<false>, blocks, lds, s, p);
 }
 
 hipError_t launch_first(const FirstPass& p, hipStream_t s) {
@@ -804,7 +884,8 @@ hipError_t launch_first(const FirstPass& p, hipStream_t s) {
   if (p.G0 == LEN) return launch_x(fft_forward_first_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_FIRST_CASE)
 #undef GNUFFT_FIRST_CASE
-  return launch_x(fft_forward_first_kernel, blocks, lds, s, p);
+  if (prime_radix(p.G0)) return launch_x(fft_forward_first_kernel<true>, blocks, lds, s, p);
+  return launch_x(fft_forward_first_kernel<false>, blocks, lds, s, p);
 }
 
 // Pass descriptors of the adjoint: grid [coils][G2][G1][G0] (the
@@ -983,6 +1064,13 @@ bool fused_fft_preferred(const Geom& g) {
   if (!fused_fft_supported(g)) return false;
   for (int a = 0; a < g.dim; a++)
     if (16 * pad_of(g.G[a]) > kBigTileElems) return false;
+  return true;
+}
+
+bool fused_fft_batchable(const Geom& g) {
+  if (!fused_fft_supported(g)) return false;
+  for (int a = 0; a < g.dim; a++)
+    if (prime_radix(g.G[a])) return false;
   return true;
 }
 

@@ -322,6 +322,8 @@ hipError_t launch_pre_batch(const BatchOp* ops, const BatchList& l, hipStream_t 
 // Fused oversampled-grid FFT (fft.hip).
 bool fused_fft_supported(const Geom& g);
 bool fused_fft_preferred(const Geom& g);
+// Supported by the batched fused FFT (no radix 7 .. 19 stage).
+bool fused_fft_batchable(const Geom& g);
 size_t fused_fft_tmp_elems(const Geom& g);
 std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][3]);
 hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,

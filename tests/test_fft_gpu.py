@@ -9,8 +9,8 @@ and forward on one plan with both FFT kinds; both must equal the oracle
 (rel-L2 <= 1e-5) and each other. The grid lengths cover even and odd
 lengths, radix 2/3/4/5/8 mixes, the lengths with compile-time stage tables
 (96, 128, 160, 192, 256), lines above 511 points (large LDS tiles),
-anisotropic 3-D grids and a length with a factor 7 (rocFFT only:
-set_fft(FUSED) is refused).
+anisotropic 3-D grids, the prime radices 7, 11, 13, 17 and 19, and a
+length with a factor 23 (rocFFT only: set_fft(FUSED) is refused).
 """
 
 import numpy as np
@@ -35,6 +35,12 @@ CASES = [
     ((20, 24, 16), 1.5),    # 30 x 36 x 24
     ((16, 16, 16), 2.0),    # 32^3
     ((25, 16, 10), 2.0),    # 50 x 32 x 20
+    ((28, 28), 1.5),        # 42 = 2 * 3 * 7 (radix 7)
+    ((44, 26), 2.0),        # 88 x 52: radix 11, 13
+    ((68, 76), 2.0),        # 136 x 152: radix 17, 19
+    ((56, 52, 34), 2.0),    # 112 x 104 x 68: radix 7, 13, 17 in 3-D
+    ((152, 152), 2.0),      # 304 = 16 * 19 (the serving mix's 3-D length, here 2-D)
+    ((340, 100), 2.0),      # 680 = 8 * 5 * 17: a large tile with a prime stage
 ]
 
 
@@ -99,7 +105,7 @@ def test_fused_equals_rocfft_and_oracle(oracle, env, dims, osf, with_sens):
 
 def test_unsupported_length_uses_rocfft(oracle, env):
     torch, capi = env
-    dims, osf = (28, 28), 1.5  # 42 = 2 * 3 * 7
+    dims, osf = (46, 46), 2.0  # 92 = 4 * 23
     p = capi.Plan(dims, 3, 8, osf)
     assert p.fft() == capi.FFT_ROCFFT
     with pytest.raises(capi.GnufftError) as e:
@@ -109,7 +115,7 @@ def test_unsupported_length_uses_rocfft(oracle, env):
     p.set_trajectory(_dev(torch, traj))
     rng = np.random.default_rng(3)
     kd = W.complex_normal(rng, (2, traj.shape[1]))
-    out = torch.empty((2, 28, 28), dtype=torch.complex64, device="cuda")
+    out = torch.empty((2, 46, 46), dtype=torch.complex64, device="cuda")
     p.adjoint(_dev(torch, kd), out, 2)
     torch.cuda.synchronize()
     op = oracle.Operator(traj, dims, 3, 8, osf, order="stable")
