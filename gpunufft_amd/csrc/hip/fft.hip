@@ -134,9 +134,10 @@ __device__ __forceinline__ void dft<5>(float2* v, float sign) {
 // X_k = x_0 + sum_n (x_n + x_{R-n}) cos(2 pi n k / R)
 //           + i sum_n (x_n - x_{R-n}) sign sin(2 pi n k / R), n = 1 .. (R-1)/2.
 // The constants come from the stage's twiddle table: w[m * step] =
-// exp(sign 2 pi i m / R) for a line of step * R points.
-template <int R>
-__device__ __forceinline__ void dft_prime(float2* v, const float2* w, int step) {
+// exp(sign 2 pi i m / R) for a line of step * R points. Each output goes to
+// out(k, X_k) as soon as it is formed (no output array: fewer registers).
+template <int R, typename Out>
+__device__ __forceinline__ void dft_prime_out(const float2* v, const float2* w, int step, Out out) {
   constexpr int H = (R - 1) / 2;
   float c[H], sn[H];
 #pragma unroll
@@ -154,7 +155,7 @@ __device__ __forceinline__ void dft_prime(float2* v, const float2* w, int step) 
     u[n - 1] = csub(v[n], v[R - n]);
     sum = cadd(sum, t[n - 1]);
   }
-  v[0] = sum;
+  out(0, sum);
 #pragma unroll
   for (int k = 1; k <= H; k++) {
     float2 a = x0, b = make_float2(0.f, 0.f);
@@ -166,19 +167,9 @@ __device__ __forceinline__ void dft_prime(float2* v, const float2* w, int step) 
       a = make_float2(a.x + t[n - 1].x * cm, a.y + t[n - 1].y * cm);
       b = make_float2(b.x + u[n - 1].x * sm, b.y + u[n - 1].y * sm);
     }
-    v[k] = make_float2(a.x - b.y, a.y + b.x);      // a + i b
-    v[R - k] = make_float2(a.x + b.y, a.y - b.x);  // a - i b
+    out(k, make_float2(a.x - b.y, a.y + b.x));
+    out(R - k, make_float2(a.x + b.y, a.y - b.x));
   }
-}
-
-// The DFT of one stage: closed forms for 2, 3, 4, 5, 8; the direct form for
-// the larger primes (constants from the twiddles w of the line).
-template <int R>
-__device__ __forceinline__ void dft_stage(float2* v, float sign, const float2* w, int step) {
-  if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
-    dft<R>(v, sign);
-  else
-    dft_prime<R>(v, w, step);
 }
 
 // n / d for n * d < 2^32 via a multiply-high with m = ceil(2^32 / d); d == 1
@@ -234,10 +225,15 @@ __device__ __forceinline__ void finish_butterfly(float2* v, int j, int l, const 
 #pragma unroll
     for (int s = 1; s < R; s++) v[s] = cmul(v[s], tw[k * s * st.tws]);
   }
-  dft_stage<R>(v, sign, tw, st.per_line);
   const int base = jq * st.Ns * R + k;
+  if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
+    dft<R>(v, sign);
 #pragma unroll
-  for (int s = 0; s < R; s++) put(l, base + s * st.Ns, v[s]);
+    for (int s = 0; s < R; s++) put(l, base + s * st.Ns, v[s]);
+  } else {
+    dft_prime_out<R>(v, tw, st.per_line,
+                     [&](int s, float2 x) { put(l, base + s * st.Ns, x); });
+  }
 }
 
 // Stage with its inputs fetched by get(l, index) and outputs stored by
