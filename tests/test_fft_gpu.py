@@ -40,7 +40,8 @@ CASES = [
     ((68, 76), 2.0),        # 136 x 152: radix 17, 19
     ((56, 52, 34), 2.0),    # 112 x 104 x 68: radix 7, 13, 17 in 3-D
     ((152, 152), 2.0),      # 304 = 16 * 19 (the serving mix's 3-D length, here 2-D)
-    ((340, 100), 2.0),      # 680 = 8 * 5 * 17: a large tile with a prime stage
+    ((340, 100), 2.0),      # 680 = 8 * 5 * 17 along x (the x passes)
+    ((100, 340), 2.0),      # 680 along y: a large (1024-thread) tile with a prime stage
 ]
 
 
