@@ -181,7 +181,6 @@ struct gnufft_plan_s {
   std::vector<float> lut;
   float* lut_d = nullptr;
   float* deapo_d = nullptr;
-  std::vector<float> deapo_h;
   // trajectory tables
   uint32_t M = 0;
   float* traj_sorted_d = nullptr;
@@ -283,7 +282,9 @@ void free_ffts(gnufft_plan p) {
 // axis by axis in double (the phases are separable), which reproduces the
 // reference's values also where |K| is tiny and the rounding of the tap
 // products is amplified (osf near 1). deapo = sqrt(Ng/Ni)/|K(p)|: the
-// compensation the reference's observed outputs correspond to.
+// compensation the reference's observed outputs correspond to. The x and y
+// sums run on the host (tap count x image row / plane), the z sum and the
+// modulus per pixel on the device.
 int compute_deapo(gnufft_plan p, hipStream_t s) {
   Geom d = p->g;  // same table, radius and anisotropy; sector width 8
   d.sw = 8;
@@ -347,19 +348,20 @@ int compute_deapo(gnufft_plan p, hipStream_t s) {
           acc += A[((size_t)tz * T[1] + ty) * Nx + x] * E[1][(size_t)ty * Ny + y];
         B[((size_t)tz * Ny + y) * Nx + x] = acc;
       }
+  // the z axis on the device: Ni points (3.5 M at 152^3) would take tens
+  // of milliseconds of host time per plan
   const double norm = std::sqrt((double)d.Ng / (double)d.Ni);
-  std::vector<float> deapo((size_t)d.Ni);
-  for (int z = 0; z < Nz; z++)
-    for (int y = 0; y < Ny; y++)
-      for (int x = 0; x < Nx; x++) {
-        cd acc(0.0, 0.0);
-        for (int tz = 0; tz < T[2]; tz++)
-          acc += B[((size_t)tz * Ny + y) * Nx + x] * E[2][(size_t)tz * Nz + z];
-        deapo[((size_t)z * Ny + y) * Nx + x] = (float)(norm / std::abs(acc));
-      }
-  HIP_TRY(hipMemcpyAsync(p->deapo_d, deapo.data(), deapo.size() * sizeof(float),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  static_assert(sizeof(cd) == sizeof(double2), "complex<double> is two doubles");
+  double2* tab = nullptr;
+  const size_t nb = B.size(), ne = E[2].size();
+  HIP_TRY(hipMalloc(&tab, (nb + ne) * sizeof(double2)));
+  hipError_t e = hipMemcpyAsync(tab, B.data(), nb * sizeof(double2), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(tab + nb, E[2].data(), ne * sizeof(double2), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_deapo_combine(tab, tab + nb, T[2], Nx, Ny, Nz, norm, p->deapo_d, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // host tables B, E go out of scope
+  (void)hipFree(tab);
+  HIP_TRY(e);
   return GNUFFT_OK;
 }
 

@@ -238,6 +238,9 @@ hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStrea
 
 // prepost.hip
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// out[z][y][x] = norm / |sum_t B[t][y][x] E[t][z]| (double), t < T.
+hipError_t launch_deapo_combine(const double2* B, const double2* E, int T, int Nx, int Ny, int Nz,
+                                double norm, float* out, hipStream_t s);
 hipError_t launch_adjoint_post(const Geom& g, const float2* grid, int n_coils,
                                const float* deapo, const float2* sens, float2* out,
                                bool accumulate, int stop, hipStream_t s);

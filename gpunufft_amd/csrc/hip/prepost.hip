@@ -32,6 +32,38 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Last axis of the deapodization DFT (the plan's set-up): pixel (z, y, x)
+// = norm / |sum_t B[t][y][x] E[t][z]| in double, B the transform of the
+// separable tap tensor over x and y, E the phases along z (one row of ones
+// in 2-D).
+__global__ void __launch_bounds__(256) deapo_combine_kernel(const double2* __restrict__ B,
+                                                            const double2* __restrict__ E, int T,
+                                                            int Nx, int Ny, int Nz, double norm,
+                                                            float* __restrict__ out) {
+  const size_t plane = (size_t)Nx * Ny;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= plane * Nz) return;
+  const int z = (int)(i / plane);
+  const size_t yx = i - (size_t)z * plane;
+  double re = 0.0, im = 0.0;
+  for (int t = 0; t < T; t++) {
+    const double2 b = B[(size_t)t * plane + yx];
+    const double2 e = E[(size_t)t * Nz + z];
+    re += b.x * e.x - b.y * e.y;
+    im += b.x * e.y + b.y * e.x;
+  }
+  out[i] = (float)(norm / hypot(re, im));
+}
+
+hipError_t launch_deapo_combine(const double2* B, const double2* E, int T, int Nx, int Ny, int Nz,
+                                double norm, float* out, hipStream_t s) {
+  const size_t n = (size_t)Nx * Ny * Nz;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(deapo_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, E,
+                     T, Nx, Ny, Nz, norm, out);
+  return hipGetLastError();
+}
+
 // Image pixel -> (x, y, z) and the grid index after crop offset and the
 // INVERSE fftshift (the second shift of the adjoint, before crop).
 __device__ inline void pixel_coords(const Geom& g, long long p, int* x, int* y, int* z) {
