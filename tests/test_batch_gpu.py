@@ -2,8 +2,9 @@
 adjoint and forward equal its own single-plan adjoint and forward and the
 oracle (rel-L2 <= 1e-5), for uneven image sizes, trajectory lengths, coil
 counts, with and without sensitivities, in 2-D and 3-D, with the fused FFT
-(one launch per pass for all plans), with rocFFT (one plan per grid size)
-and with both in one batch."""
+(one launch per pass for all plans), with rocFFT (one plan per grid size),
+with both in one batch, and with plans whose grid lengths need the fused
+FFT's prime stages (rocFFT inside the batch)."""
 
 import numpy as np
 import pytest
@@ -52,8 +53,20 @@ def _ops_3d():
     return out
 
 
+def _ops_2d_primes():
+    """Grid lengths with the fused FFT's prime stages (112 = 16 * 7,
+    104 = 8 * 13) beside smooth ones: alone such plans run the fused FFT,
+    inside a batch rocFFT (the batched FFT kernels omit the prime stages)."""
+    out = []
+    for j, (N, C) in enumerate([(56, 3), (64, 2), (52, 4), (48, 1)]):
+        traj = W.spiral_2d(6 + j, 400 + 50 * j, n_turns=N / 10.0)
+        sens = W.gaussian_sens((N, N), C, seed=j) if j % 2 == 0 else None
+        out.append(dict(traj=traj, img=(N, N), C=C, sens=sens, dens=W.ramp_density(traj), sw=8))
+    return out
+
+
 @pytest.mark.parametrize("fft", ["fused", "rocfft", "mixed"])
-@pytest.mark.parametrize("make", [_ops_2d, _ops_3d], ids=["2d", "3d"])
+@pytest.mark.parametrize("make", [_ops_2d, _ops_3d, _ops_2d_primes], ids=["2d", "3d", "2d_primes"])
 def test_batch_equals_single_plans(oracle, env, make, fft):
     torch, capi = env
     specs = make()
