@@ -1,5 +1,6 @@
 // Oversampled-grid FFT with the stages around it fused, for grid lengths
-// whose prime factors are 2, 3 and 5 (other lengths use rocFFT).
+// whose prime factors are 2, 3, 5, 7, 11, 13, 17 and 19 (other lengths use
+// rocFFT).
 //
 // The reference runs, per coil, INVERSE fftshift -> cuFFT -> INVERSE fftshift
 // -> crop -> 1/sqrt(Ng) -> deapodization -> conj(S) coil sum for the adjoint
@@ -9,8 +10,9 @@
 // :493-521, :569-654, :699-793), each a separate pass over the whole grid.
 //
 // Here a d-dimensional transform is d passes, one per axis. A pass loads a
-// tile of lines into LDS, transforms them with a Stockham radix-8/4/2/3/5
-// FFT and writes them back. The passes are pruned to the data that matters:
+// tile of lines into LDS, transforms them with a Stockham FFT (radices 8, 4,
+// 2, 3, 5 in closed form, the primes 7 .. 19 as direct DFTs) and writes
+// them back. The passes are pruned to the data that matters:
 //   adjoint: the z and y passes write only the cropped image range of their
 //     axis; the last (x) pass writes image pixels and applies the shift,
 //     crop, 1/sqrt(Ng), deapodization and the conj(S) coil sum on the way out;
@@ -829,6 +831,11 @@ size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
   X(192, 32, 8, 8, 3)             \
   X(256, 16, 8, 8, 4)
 
+// Line lengths with compile-time kernels for the x passes only (their y / z
+// passes use the large tiles).
+#define GNUFFT_FIXED_X_SIZES(X) \
+  X(512, 0, 8, 8, 8)
+
 template <typename K>
 hipError_t launch_pass(K kernel, uint32_t blocks, size_t lds, hipStream_t s,
                        const AxisPass& p, int threads = kFftThreads) {
@@ -868,6 +875,7 @@ hipError_t launch_last(const LastPass& p, hipStream_t s) {
 #define GNUFFT_LAST_CASE(LEN, TILE, RA, RB, RC) \
   if (p.G0 == LEN) return launch_x(fft_adjoint_last_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_LAST_CASE)
+  GNUFFT_FIXED_X_SIZES(GNUFFT_LAST_CASE)
 #undef GNUFFT_LAST_CASE
   if (prime_radix(p.G0)) return launch_x(fft_adjoint_last_kernel<true>, blocks, lds, s, p);
   return launch_x(fft_adjoint_last_kernel<false>, blocks, lds, s, p);
@@ -879,6 +887,7 @@ hipError_t launch_first(const FirstPass& p, hipStream_t s) {
 #define GNUFFT_FIRST_CASE(LEN, TILE, RA, RB, RC) \
   if (p.G0 == LEN) return launch_x(fft_forward_first_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_FIRST_CASE)
+  GNUFFT_FIXED_X_SIZES(GNUFFT_FIRST_CASE)
 #undef GNUFFT_FIRST_CASE
   if (prime_radix(p.G0)) return launch_x(fft_forward_first_kernel<true>, blocks, lds, s, p);
   return launch_x(fft_forward_first_kernel<false>, blocks, lds, s, p);
