@@ -466,7 +466,10 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
 // ---------------------------------------------------------------------------
 constexpr size_t kInterpStaticLds = 4096;  // interp's static shared tables, rounded up
 constexpr int kFoldThreads = 256;
-constexpr int kFoldPts = 2;      // home points per thread per pass
+// home points per thread per pass: one in 3-D (8 coils x 2 points would
+// take 125 VGPRs, four waves per SIMD; one point: 71, seven), two in 2-D
+// (batches of sparse 2-D sectors are faster with two)
+__host__ __device__ constexpr int fold_pts(int dim) { return dim == 3 ? 1 : 2; }
 constexpr int kFoldCands = 256;  // candidate sectors per home sector (host-checked)
 
 // Sum every chunk sub-grid of a multi-chunk sector into the slot of its
@@ -557,9 +560,9 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
   // full block of an even coil count: every point's CB coils start on a
   // 16-byte boundary
   const bool vec = CB % 2 == 0 && coil0 + CB <= a.n_coils && a.n_coils % 2 == 0;
-  for (int base = 0; base < HP; base += nt * kFoldPts) {
+  for (int base = 0; base < HP; base += nt * fold_pts(DIM)) {
 #pragma unroll
-    for (int p = 0; p < kFoldPts; p++) {
+    for (int p = 0; p < fold_pts(DIM); p++) {
       const int e = base + p * nt + tid;
       if (e >= HP) continue;
       const int tx = e % hx;
@@ -699,12 +702,12 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
 
   const int nt = (int)blockDim.x;
   const size_t kstride = (size_t)a.n_coils * PD;
-  for (int base = 0; base < HP; base += nt * kFoldPts) {
-    int q[kFoldPts][3];
-    bool live[kFoldPts];
-    float ax[kFoldPts][CB], ay[kFoldPts][CB];
+  for (int base = 0; base < HP; base += nt * fold_pts(DIM)) {
+    int q[fold_pts(DIM)][3];
+    bool live[fold_pts(DIM)];
+    float ax[fold_pts(DIM)][CB], ay[fold_pts(DIM)][CB];
 #pragma unroll
-    for (int p = 0; p < kFoldPts; p++) {
+    for (int p = 0; p < fold_pts(DIM); p++) {
       const int e = base + p * nt + tid;
       live[p] = e < HP;
       const int lx = e % hx;
@@ -727,7 +730,7 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
       const uint32_t rp = reps_of[it];
       const float2* src = a.scratch + (size_t)v.w * kstride + coil0;  // [chunk][point][coil]
 #pragma unroll
-      for (int p = 0; p < kFoldPts; p++) {
+      for (int p = 0; p < fold_pts(DIM); p++) {
         const int ix = wrap(q[p][0] - v.x, g.G[0]) + (int)(rp & 255u) * g.G[0];
         const int iy = wrap(q[p][1] - v.y, g.G[1]) + (int)((rp >> 8) & 255u) * g.G[1];
         const int iz = DIM == 3 ? wrap(q[p][2] - v.z, g.G[2]) + (int)(rp >> 16) * g.G[2] : 0;
@@ -743,7 +746,7 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
       }
     }
 #pragma unroll
-    for (int p = 0; p < kFoldPts; p++) {
+    for (int p = 0; p < fold_pts(DIM); p++) {
       if (!live[p]) continue;
       int o0 = q[p][0], o1 = q[p][1], o2 = q[p][2];
       if (a.fold_shift) {
@@ -1099,11 +1102,11 @@ static hipError_t interp_nx(const Geom& g, const InterpArgs& a, int chunk, hipSt
   }
 }
 
-// Fold workgroup size: enough threads for kFoldPts home points each and one
+// Fold workgroup size: enough threads for fold_pts home points each and one
 // per candidate sector while the item list is built; 0 if too many candidates.
 int fold_threads(const Geom& g, const Cover& cover) {
   const int hp = g.dim == 3 ? g.sw * g.sw * g.sw : g.sw * g.sw;
-  const int per = (hp + kFoldPts - 1) / kFoldPts;
+  const int per = (hp + fold_pts(g.dim) - 1) / fold_pts(g.dim);
   int threads = per >= kFoldThreads ? kFoldThreads : ((per + 63) / 64) * 64;
   int cands = 1;
   for (int d = 0; d < g.dim; d++) cands *= cover.max_len[d] * ((g.P + g.G[d] - 1) / g.G[d]);
