@@ -861,6 +861,7 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
   // gix[0][x] + gix[1][y] + gix[2][z]
   __shared__ int gix[3][256];
   // round 0's coordinates first: their loads overlap the table set-up
+  // (p0 then carries each next round's coordinates, loaded a round ahead)
   float p0[3] = {0.0f, 0.0f, 0.0f};
   if (tid < min(R, n)) load_coord<DIM>(a.crds, g.M, ch.begin + tid, p0);
   copy_to_lds<NT, 4>(lut, a.lut, g.K);
@@ -913,13 +914,13 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
     const int rn = min(R, n - r0);
     if (r0 > 0) {
       __syncthreads();  // previous round consumed
-      for (int i = tid; i < rn; i += NT) {
-        float p[3];
-        load_coord<DIM>(a.crds, g.M, ch.begin + r0 + i, p);
-        make_record<DIM, NX>(g, lut, kp, KPS, c3, p, rec + i * RW);
-      }
+      // sample tid of this round (R <= NT); its coordinates were loaded
+      // during the previous round's taps
+      if (tid < rn) make_record<DIM, NX>(g, lut, kp, KPS, c3, p0, rec + tid * RW);
     }
     __syncthreads();  // sub-grid staged, the round's records ready
+    // the next round's coordinates, in flight during this round's taps
+    if (r0 + R + tid < n) load_coord<DIM>(a.crds, g.M, ch.begin + r0 + R + tid, p0);
 
     // consecutive samples per wave step: the scattered stores of one wave
     // cover runs of neighbouring k-space samples
