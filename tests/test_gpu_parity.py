@@ -228,7 +228,8 @@ def _adjointness(torch_dev, pl, img_shape, C, M, use_sens, seed):
 
 @pytest.mark.parametrize("cfg", ["radial2d_256_mc16", "koosh3d_128_mc8"])
 def test_full_size_properties(torch_dev, capi, cfg):
-    """Full BASELINE sizes: adjointness and linearity (size-independent checks)."""
+    """Full workload sizes (radial 256^2 with 16 coils, koosh 128^3 with 8
+    coils): adjointness and linearity (size-independent checks)."""
     w = W.config(cfg)
     print("config", cfg, flush=True)
     torch = torch_dev
