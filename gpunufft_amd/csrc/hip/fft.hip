@@ -823,13 +823,23 @@ uint32_t first_blocks(const FirstPass& p) { return (uint32_t)((p.lines + p.Q - 1
 size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
 
 // Line lengths with compile-time kernels: (L, tile lines of a y / z pass,
-// radices). The tile must be the one finish_axis picks for L.
+// radices). The tile must be the one finish_axis picks for L. Besides the
+// common smooth lengths, the 3-D grids of 104^3 .. 152^3 images at osf 1.5
+// and 2 with a prime factor 7 .. 19.
 #define GNUFFT_FIXED_FFT_SIZES(X) \
   X(96, 32, 8, 4, 3)              \
   X(128, 32, 8, 8, 2)             \
   X(160, 32, 8, 4, 5)             \
   X(192, 32, 8, 8, 3)             \
-  X(256, 16, 8, 8, 4)
+  X(256, 16, 8, 8, 4)              \
+  X(156, 32, 4, 3, 13)             \
+  X(168, 32, 8, 3, 7)              \
+  X(204, 32, 4, 3, 17)             \
+  X(208, 32, 8, 2, 13)             \
+  X(224, 32, 8, 4, 7)              \
+  X(228, 32, 4, 3, 19)             \
+  X(272, 16, 8, 2, 17)             \
+  X(304, 16, 8, 2, 19)
 
 // Line lengths with compile-time kernels for the x passes only (their y / z
 // passes use the large tiles).

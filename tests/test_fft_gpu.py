@@ -8,7 +8,8 @@ full adjoint (stops FFT and DEAPODIZATION, with and without sensitivities)
 and forward on one plan with both FFT kinds; both must equal the oracle
 (rel-L2 <= 1e-5) and each other. The grid lengths cover even and odd
 lengths, radix 2/3/4/5/8 mixes, the lengths with compile-time stage tables
-(96, 128, 160, 192, 256), lines above 511 points (large LDS tiles),
+(96, 128, 160, 192, 256 and eight lengths with a prime factor 7 .. 19),
+lines above 511 points (large LDS tiles),
 anisotropic 3-D grids, the prime radices 7, 11, 13, 17 and 19, and a
 length with a factor 23 (rocFFT only: set_fft(FUSED) is refused).
 """
@@ -42,6 +43,11 @@ CASES = [
     ((152, 152), 2.0),      # 304 = 16 * 19 (the serving mix's 3-D length, here 2-D)
     ((340, 100), 2.0),      # 680 = 8 * 5 * 17 along x (the x passes)
     ((100, 340), 2.0),      # 680 along y: a large (1024-thread) tile with a prime stage
+    ((104, 152), 2.0),      # 208 x 304: compile-time stages with radix 13 / 19
+    ((152, 104), 2.0),      # 304 x 208
+    ((114, 136), 2.0),      # 228 x 272: compile-time radix 19 / 17
+    ((112, 84), 2.0),       # 224 x 168: compile-time radix 7
+    ((78, 102), 2.0),       # 156 x 204: compile-time radix 13 / 17
 ]
 
 
