@@ -824,8 +824,8 @@ size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
 
 // Line lengths with compile-time kernels: (L, tile lines of a y / z pass,
 // radices). The tile must be the one finish_axis picks for L. Besides the
-// common smooth lengths, the 3-D grids of 104^3 .. 152^3 images at osf 1.5
-// and 2 with a prime factor 7 .. 19.
+// common smooth lengths (and 320, the 160^3 osf-2 grid), the 3-D grids of
+// 104^3 .. 152^3 images at osf 1.5 and 2 with a prime factor 7 .. 19.
 #define GNUFFT_FIXED_FFT_SIZES(X) \
   X(96, 32, 8, 4, 3)              \
   X(128, 32, 8, 8, 2)             \
@@ -839,7 +839,8 @@ size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
   X(224, 32, 8, 4, 7)              \
   X(228, 32, 4, 3, 19)             \
   X(272, 16, 8, 2, 17)             \
-  X(304, 16, 8, 2, 19)
+  X(304, 16, 8, 2, 19)             \
+  X(320, 16, 8, 8, 5)
 
 // Line lengths with compile-time kernels for the x passes only (their y / z
 // passes use the large tiles).

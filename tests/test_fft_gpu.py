@@ -48,6 +48,7 @@ CASES = [
     ((114, 136), 2.0),      # 228 x 272: compile-time radix 19 / 17
     ((112, 84), 2.0),       # 224 x 168: compile-time radix 7
     ((78, 102), 2.0),       # 156 x 204: compile-time radix 13 / 17
+    ((160, 40), 2.0),       # 320 x 80: compile-time 8 * 8 * 5
 ]
 
 
