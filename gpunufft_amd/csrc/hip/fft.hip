@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "gnufft_internal.hpp"
@@ -316,13 +317,35 @@ constexpr uint32_t qmagic(int d) {
   return d == 1 ? 0u : (uint32_t)((0x100000000ull + (unsigned)d - 1) / (unsigned)d);
 }
 
-template <int L, int R0, int R1, int R2>
+template <int L, int... R>
 struct FixedStages {
-  static_assert(R0 * R1 * R2 == L, "radices multiply to the line length");
-  static constexpr FftStage s0{R0, 1, L / R0, L / R0, 0u, 0u};
-  static constexpr FftStage s1{R1, R0, L / R1, L / (R0 * R1), qmagic(R0), 0u};
-  static constexpr FftStage s2{R2, R0 * R1, L / R2, 1, qmagic(R0 * R1), 0u};
+  static constexpr int n = sizeof...(R);
+  static constexpr int rad[n] = {R...};
+  static_assert((R * ...) == L, "radices multiply to the line length");
+  static_assert(n >= 2, "first and last stages are distinct");
+  // points of the sub-transforms that stage i combines
+  static constexpr int ns(int i) { return i == 0 ? 1 : ns(i - 1) * rad[i - 1]; }
+  template <int I>
+  static constexpr FftStage s{rad[I], ns(I), L / rad[I], L / (ns(I) * rad[I]), qmagic(ns(I)), 0u};
 };
+
+// Stages I... of F over `lines` lines, each followed by a barrier.
+template <typename F, int... I, typename Get, typename Put>
+__device__ __forceinline__ void fixed_stages(int lines, uint32_t m_lines, const float2* tw, float sign,
+                                             bool sync, Get get, Put put,
+                                             std::integer_sequence<int, I...>) {
+  ((stage<F::rad[I]>(lines, m_lines, F::template s<I>, tw, sign, sync, get, put), __syncthreads()),
+   ...);
+}
+
+// Stages 1 .. n-2 of F (the in-LDS stages between a global first and last).
+template <typename F, int... I, typename Get, typename Put>
+__device__ __forceinline__ void fixed_middle(int lines, uint32_t m_lines, const float2* tw, float sign,
+                                             Get get, Put put, std::integer_sequence<int, I...>) {
+  ((stage<F::rad[I + 1]>(lines, m_lines, F::template s<I + 1>, tw, sign, true, get, put),
+    __syncthreads()),
+   ...);
+}
 
 // Stage runners: all stages of `lines` LDS lines (the x passes).
 template <bool PR>
@@ -335,19 +358,15 @@ struct RunTable {
   }
 };
 
-template <int L, int R0, int R1, int R2>
+template <int L, int... R>
 struct RunFixed {
   __device__ __forceinline__ void operator()(float2* buf, int lines, uint32_t m_lines, int LP,
                                              const float2* tw, float sign) const {
-    using F = FixedStages<L, R0, R1, R2>;
+    using F = FixedStages<L, R...>;
     auto get = [&](int l, int i) { return buf[l * LP + i]; };
     auto put = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
-    stage<R0>(lines, m_lines, F::s0, tw, sign, true, get, put);
-    __syncthreads();
-    stage<R1>(lines, m_lines, F::s1, tw, sign, true, get, put);
-    __syncthreads();
-    stage<R2>(lines, m_lines, F::s2, tw, sign, true, get, put);
-    __syncthreads();
+    fixed_stages<F>(lines, m_lines, tw, sign, true, get, put,
+                    std::make_integer_sequence<int, F::n>{});
   }
 };
 
@@ -443,11 +462,11 @@ __global__ void __launch_bounds__(kBigThreads, 1) fft_axis_big_kernel(AxisPass p
   fft_axis_body<kBigThreads, PR>(p, blockIdx.x);
 }
 
-// The same pass for a line length L = R0 R1 R2 and tile of S lines known at
-// compile time (S a power of two).
-template <int L, int S, int R0, int R1, int R2>
+// The same pass for a line length L = R0 R1 ... (two or more stages) and a
+// tile of S lines known at compile time (S a power of two).
+template <int L, int S, int... R>
 __device__ __forceinline__ void fft_axis_fixed_body(const AxisPass& p, int block) {
-  using F = FixedStages<L, R0, R1, R2>;
+  using F = FixedStages<L, R...>;
   constexpr int LP = (L + 1) | 1;
   constexpr uint32_t mS = qmagic(S);
   extern __shared__ float2 lds[];
@@ -478,16 +497,16 @@ __device__ __forceinline__ void fft_axis_fixed_body(const AxisPass& p, int block
   auto get_lds = [&](int l, int i) { return buf[l * LP + i]; };
   auto put_lds = [&](int l, int i, float2 v) { buf[l * LP + i] = v; };
   __syncthreads();  // twiddles
-  stage<R0>(S, mS, F::s0, tw, p.sign, false, get_global, put_lds);
+  constexpr int n = F::n;
+  stage<F::rad[0]>(S, mS, F::template s<0>, tw, p.sign, false, get_global, put_lds);
   __syncthreads();
-  stage<R1>(S, mS, F::s1, tw, p.sign, true, get_lds, put_lds);
-  __syncthreads();
-  stage<R2>(S, mS, F::s2, tw, p.sign, false, get_lds, put_global);
+  fixed_middle<F>(S, mS, tw, p.sign, get_lds, put_lds, std::make_integer_sequence<int, n - 2>{});
+  stage<F::rad[n - 1]>(S, mS, F::template s<n - 1>, tw, p.sign, false, get_lds, put_global);
 }
 
-template <int L, int S, int R0, int R1, int R2>
+template <int L, int S, int... R>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_fixed_kernel(AxisPass p) {
-  fft_axis_fixed_body<L, S, R0, R1, R2>(p, blockIdx.x);
+  fft_axis_fixed_body<L, S, R...>(p, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,9 +689,9 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_kernel(LastPa
   fft_adjoint_last_body(p, blockIdx.x, RunTable<PR>{p.st, p.n_st});
 }
 
-template <int L, int R0, int R1, int R2>
+template <int L, int... R>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_fixed_kernel(LastPass p) {
-  fft_adjoint_last_body(p, blockIdx.x, RunFixed<L, R0, R1, R2>{});
+  fft_adjoint_last_body(p, blockIdx.x, RunFixed<L, R...>{});
 }
 
 template <bool PR>
@@ -680,9 +699,9 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_kernel(First
   fft_forward_first_body(p, blockIdx.x, RunTable<PR>{p.st, p.n_st});
 }
 
-template <int L, int R0, int R1, int R2>
+template <int L, int... R>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_fixed_kernel(FirstPass p) {
-  fft_forward_first_body(p, blockIdx.x, RunFixed<L, R0, R1, R2>{});
+  fft_forward_first_body(p, blockIdx.x, RunFixed<L, R...>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -825,22 +844,28 @@ size_t first_lds(const FirstPass& p) { return tile_bytes(p.G0, p.Q * p.Cg); }
 // Line lengths with compile-time kernels: (L, tile lines of a y / z pass,
 // radices). The tile must be the one finish_axis picks for L. Besides the
 // common smooth lengths (and 320, the 160^3 osf-2 grid), the 3-D grids of
-// 104^3 .. 152^3 images at osf 1.5 and 2 with a prime factor 7 .. 19.
+// 96^3 .. 160^3 images at osf 1.5 and 2: those with a prime factor 7 .. 19
+// and the smooth ones that need four stages (144, 180, 216, 240, 288).
 #define GNUFFT_FIXED_FFT_SIZES(X) \
   X(96, 32, 8, 4, 3)              \
   X(128, 32, 8, 8, 2)             \
   X(160, 32, 8, 4, 5)             \
   X(192, 32, 8, 8, 3)             \
-  X(256, 16, 8, 8, 4)              \
-  X(156, 32, 4, 3, 13)             \
-  X(168, 32, 8, 3, 7)              \
-  X(204, 32, 4, 3, 17)             \
-  X(208, 32, 8, 2, 13)             \
-  X(224, 32, 8, 4, 7)              \
-  X(228, 32, 4, 3, 19)             \
-  X(272, 16, 8, 2, 17)             \
-  X(304, 16, 8, 2, 19)             \
-  X(320, 16, 8, 8, 5)
+  X(256, 16, 8, 8, 4)             \
+  X(156, 32, 4, 3, 13)            \
+  X(168, 32, 8, 3, 7)             \
+  X(204, 32, 4, 3, 17)            \
+  X(208, 32, 8, 2, 13)            \
+  X(224, 32, 8, 4, 7)             \
+  X(228, 32, 4, 3, 19)            \
+  X(272, 16, 8, 2, 17)            \
+  X(304, 16, 8, 2, 19)            \
+  X(320, 16, 8, 8, 5)             \
+  X(144, 32, 4, 4, 3, 3)          \
+  X(180, 32, 4, 5, 3, 3)          \
+  X(216, 32, 8, 3, 3, 3)          \
+  X(240, 32, 4, 4, 5, 3)          \
+  X(288, 16, 8, 4, 3, 3)
 
 // Line lengths with compile-time kernels for the x passes only (their y / z
 // passes use the large tiles).
@@ -859,9 +884,9 @@ hipError_t launch_pass(K kernel, uint32_t blocks, size_t lds, hipStream_t s,
 hipError_t launch_axis(const AxisPass& p, hipStream_t s) {
   const size_t lds = axis_lds(p);
   const uint32_t blocks = axis_blocks(p);
-#define GNUFFT_AXIS_CASE(LEN, TILE, RA, RB, RC)    \
-  if (p.L == LEN && p.stripe == TILE)              \
-    return launch_pass(fft_axis_fixed_kernel<LEN, TILE, RA, RB, RC>, blocks, lds, s, p);
+#define GNUFFT_AXIS_CASE(LEN, TILE, ...) \
+  if (p.L == LEN && p.stripe == TILE)    \
+    return launch_pass(fft_axis_fixed_kernel<LEN, TILE, __VA_ARGS__>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_AXIS_CASE)
 #undef GNUFFT_AXIS_CASE
   if (prime_radix(p.L)) {
@@ -883,8 +908,8 @@ hipError_t launch_x(K kernel, uint32_t blocks, size_t lds, hipStream_t s, const 
 hipError_t launch_last(const LastPass& p, hipStream_t s) {
   const size_t lds = last_lds(p);
   const uint32_t blocks = last_blocks(p);
-#define GNUFFT_LAST_CASE(LEN, TILE, RA, RB, RC) \
-  if (p.G0 == LEN) return launch_x(fft_adjoint_last_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
+#define GNUFFT_LAST_CASE(LEN, TILE, ...) \
+  if (p.G0 == LEN) return launch_x(fft_adjoint_last_fixed_kernel<LEN, __VA_ARGS__>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_LAST_CASE)
   GNUFFT_FIXED_X_SIZES(GNUFFT_LAST_CASE)
 #undef GNUFFT_LAST_CASE
@@ -895,8 +920,8 @@ hipError_t launch_last(const LastPass& p, hipStream_t s) {
 hipError_t launch_first(const FirstPass& p, hipStream_t s) {
   const size_t lds = first_lds(p);
   const uint32_t blocks = first_blocks(p);
-#define GNUFFT_FIRST_CASE(LEN, TILE, RA, RB, RC) \
-  if (p.G0 == LEN) return launch_x(fft_forward_first_fixed_kernel<LEN, RA, RB, RC>, blocks, lds, s, p);
+#define GNUFFT_FIRST_CASE(LEN, TILE, ...) \
+  if (p.G0 == LEN) return launch_x(fft_forward_first_fixed_kernel<LEN, __VA_ARGS__>, blocks, lds, s, p);
   GNUFFT_FIXED_FFT_SIZES(GNUFFT_FIRST_CASE)
   GNUFFT_FIXED_X_SIZES(GNUFFT_FIRST_CASE)
 #undef GNUFFT_FIRST_CASE

@@ -49,6 +49,10 @@ CASES = [
     ((112, 84), 2.0),       # 224 x 168: compile-time radix 7
     ((78, 102), 2.0),       # 156 x 204: compile-time radix 13 / 17
     ((160, 40), 2.0),       # 320 x 80: compile-time 8 * 8 * 5
+    ((96, 120), 1.5),       # 144 x 180: compile-time four stages 4 * 4 * 3 * 3, 4 * 5 * 3 * 3
+    ((144, 96), 1.5),       # 216 x 144: 8 * 3 * 3 * 3 along x
+    ((120, 144), 2.0),      # 240 x 288: 4 * 4 * 5 * 3, 8 * 4 * 3 * 3 (16-line tiles)
+    ((20, 24, 108), 2.0),   # 40 x 48 x 216: a four-stage z pass
 ]
 
 
