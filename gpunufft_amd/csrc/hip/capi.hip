@@ -16,6 +16,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../../include/gpunufft_hip.h"
@@ -122,12 +123,56 @@ int host_sector_of(float coord, int G, int sw) {
 }
 #pragma clang fp contract(on)
 
+// Geometry of an in-place C2C rocFFT plan pair on one device.
+struct FftKey {
+  int device = 0, dims = 0;
+  uint32_t G0 = 0, G1 = 0, G2 = 0, batch = 0;
+  bool operator<(const FftKey& o) const {
+    return std::tie(device, dims, G0, G1, G2, batch) <
+           std::tie(o.device, o.dims, o.G0, o.G1, o.G2, o.batch);
+  }
+};
+
 struct FftPlans {
   rocfft_plan fwd = nullptr, inv = nullptr;
   rocfft_execution_info info = nullptr;
   void* work = nullptr;
   size_t work_bytes = 0;
+  FftKey key;
 };
+
+// Idle rocFFT plan pairs, kept when their owner (a plan or a batch group)
+// lets them go, so that a later owner of the same geometry on the same
+// device skips rocFFT's plan construction (milliseconds of host time). A
+// pair has one owner at a time; the pool keeps at most kFftPoolMax pairs
+// (with their twiddle tables in device memory) until the process ends.
+constexpr size_t kFftPoolMax = 32;
+std::mutex g_fft_pool_mu;
+std::multimap<FftKey, std::pair<rocfft_plan, rocfft_plan>> g_fft_pool;
+
+bool take_pooled_fft(const FftKey& key, rocfft_plan* fwd, rocfft_plan* inv) {
+  std::lock_guard<std::mutex> lk(g_fft_pool_mu);
+  auto it = g_fft_pool.find(key);
+  if (it == g_fft_pool.end()) return false;
+  *fwd = it->second.first;
+  *inv = it->second.second;
+  g_fft_pool.erase(it);
+  return true;
+}
+
+// Returns f's plan pair to the pool (or destroys it when the pool is full).
+void release_fft_plans(FftPlans& f) {
+  if (f.fwd && f.inv) {
+    std::lock_guard<std::mutex> lk(g_fft_pool_mu);
+    if (g_fft_pool.size() < kFftPoolMax) {
+      g_fft_pool.emplace(f.key, std::make_pair(f.fwd, f.inv));
+      f.fwd = f.inv = nullptr;
+    }
+  }
+  if (f.fwd) rocfft_plan_destroy(f.fwd);
+  if (f.inv) rocfft_plan_destroy(f.inv);
+  f.fwd = f.inv = nullptr;
+}
 
 // HIP-event stage timer of one plan.
 struct StageTimer {
@@ -265,8 +310,7 @@ void free_traj(gnufft_plan p) {
 
 void free_ffts(gnufft_plan p) {
   for (auto& kv : p->ffts) {
-    if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
-    if (kv.second.inv) rocfft_plan_destroy(kv.second.inv);
+    release_fft_plans(kv.second);
     if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
     dfree(kv.second.work);
   }
@@ -422,12 +466,20 @@ int create_fft(const Geom& g, uint32_t batch, FftPlans* out) {
   FftPlans f;
   size_t lengths[3] = {(size_t)g.G[0], (size_t)g.G[1], (size_t)g.G[2]};
   const size_t dims = (size_t)g.dim;
-  ROCFFT_TRY(rocfft_plan_create(&f.fwd, rocfft_placement_inplace,
-                                rocfft_transform_type_complex_forward, rocfft_precision_single,
-                                dims, lengths, batch, nullptr));
-  ROCFFT_TRY(rocfft_plan_create(&f.inv, rocfft_placement_inplace,
-                                rocfft_transform_type_complex_inverse, rocfft_precision_single,
-                                dims, lengths, batch, nullptr));
+  HIP_TRY(hipGetDevice(&f.key.device));
+  f.key.dims = g.dim;
+  f.key.G0 = g.G[0];
+  f.key.G1 = g.G[1];
+  f.key.G2 = g.G[2];
+  f.key.batch = batch;
+  if (!take_pooled_fft(f.key, &f.fwd, &f.inv)) {
+    ROCFFT_TRY(rocfft_plan_create(&f.fwd, rocfft_placement_inplace,
+                                  rocfft_transform_type_complex_forward, rocfft_precision_single,
+                                  dims, lengths, batch, nullptr));
+    ROCFFT_TRY(rocfft_plan_create(&f.inv, rocfft_placement_inplace,
+                                  rocfft_transform_type_complex_inverse, rocfft_precision_single,
+                                  dims, lengths, batch, nullptr));
+  }
   size_t w1 = 0, w2 = 0;
   ROCFFT_TRY(rocfft_plan_get_work_buffer_size(f.fwd, &w1));
   ROCFFT_TRY(rocfft_plan_get_work_buffer_size(f.inv, &w2));
@@ -440,8 +492,7 @@ int create_fft(const Geom& g, uint32_t batch, FftPlans* out) {
 }
 
 void destroy_fft(FftPlans& f) {
-  if (f.fwd) rocfft_plan_destroy(f.fwd);
-  if (f.inv) rocfft_plan_destroy(f.inv);
+  release_fft_plans(f);
   if (f.info) rocfft_execution_info_destroy(f.info);
   if (f.work) (void)hipFree(f.work);
   f = FftPlans();

@@ -134,6 +134,47 @@ def test_unsupported_length_uses_rocfft(oracle, env):
     assert oracle.rel_l2(out.cpu().numpy(), op.adjoint(kd)) < TOL
 
 
+def test_rocfft_plan_pairs_are_reused(oracle, env):
+    """A destroyed plan's rocFFT plans go back to a process-wide pool and
+    serve the next plan of the same geometry and coil batch; two live plans
+    of one geometry each hold their own pair. Every result equals the
+    oracle, for both directions, across reuse and with both plans live."""
+    torch, capi = env
+    dims, osf, C = (46, 46), 2.0, 2  # 92 = 4 * 23: rocFFT
+    traj = _traj(dims, 5)
+    rng = np.random.default_rng(11)
+    kd = W.complex_normal(rng, (C, traj.shape[1]))
+    img = W.complex_normal(rng, (C, 46, 46))
+    op = oracle.Operator(traj, dims, 3, 8, osf, order="stable")
+    ref_a, ref_f = op.adjoint(kd), op.forward(img, C)
+
+    def make():
+        p = capi.Plan(dims, 3, 8, osf)
+        assert p.fft() == capi.FFT_ROCFFT
+        p.set_trajectory(_dev(torch, traj))
+        return p
+
+    def check(p):
+        a = torch.empty((C, 46, 46), dtype=torch.complex64, device="cuda")
+        f = torch.empty((C, traj.shape[1]), dtype=torch.complex64, device="cuda")
+        p.adjoint(_dev(torch, kd), a, C)
+        p.forward(_dev(torch, img), f, C)
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(a.cpu().numpy(), ref_a) < TOL
+        assert oracle.rel_l2(f.cpu().numpy(), ref_f) < TOL
+
+    for _ in range(3):  # create, use, destroy: later rounds take pooled plans
+        p = make()
+        check(p)
+        p.close()
+    p, q = make(), make()  # both live: one pooled pair, one new
+    check(p)
+    check(q)
+    check(p)
+    p.close()
+    q.close()
+
+
 def test_fused_headline_geometry_matches_rocfft(env):
     """koosh 128^3, osf 1.5 (192^3 grid), 8 coils with sensitivities: the
     fused and rocFFT paths agree on the full adjoint and forward."""
