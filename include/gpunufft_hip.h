@@ -104,6 +104,10 @@ int gnufft_mem_info(size_t* free_bytes, size_t* total_bytes); /* checkMemoryCons
  * img_dims[2] == 0 selects 2-D. */
 int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t kernel_width,
                        uint32_t sector_width, float osf, int lookup, void* stream);
+/* Frees the plan's device memory. Its rocFFT plans, if any, go to a
+ * process-wide pool of at most 32 idle pairs (keyed by device, grid and
+ * coil batch), from which a later plan of the same geometry takes them.
+ * The batch destroy does the same. */
 int gnufft_plan_destroy(gnufft_plan plan);
 int gnufft_plan_get_info(gnufft_plan plan, gnufft_plan_info* info);
 
