@@ -331,7 +331,9 @@ def bench_serving(args):
 
     reqs = serve.serving_mix(n_requests=args.requests, rate_hz=args.rate, seed=7)
     # first-use costs outside the measurement: every geometry of the mix once
-    # (rocFFT compiles kernels for lengths it first sees, code objects load)
+    # (rocFFT compiles kernels for lengths it first sees, code objects load,
+    # and the rocFFT plan pairs of destroyed plans stay pooled for later
+    # plans of the same geometry, as in a server that has seen them before)
     warm = serve.serving_mix(n_requests=args.requests, rate_hz=args.rate, seed=7, iterations=1)
     for r in warm:
         r.arrival = 0.0
