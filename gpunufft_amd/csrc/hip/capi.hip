@@ -1212,6 +1212,7 @@ struct gnufft_batch_s {
   // (plan, block) per pass; the other plans: rocFFT per grid size with the
   // pre / post kernels
   bool fused = false;            // any plan on the fused FFT
+  bool fft_prime = false;        // a fused plan needs the prime stages 7 .. 19
   std::vector<char> op_fused;
   std::vector<char> fft_desc_h;
   void* fft_desc_d = nullptr;
@@ -1281,14 +1282,18 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
   }
   gnufft_batch b = new (std::nothrow) gnufft_batch_s();
   if (!b) return fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: host allocation");
-  // plans on the fused FFT run it inside the batch unless a line length
-  // needs the prime-radix stages (the batch kernels omit them): those take
-  // rocFFT and the separate pre / post kernels here
-  auto batch_fused = [](const gnufft_plan q) { return q->fused && fused_fft_batchable(q->g); };
+  // plans on the fused FFT run it inside the batch (with the kernels'
+  // prime-stage variants when some plan needs them); the others take rocFFT
+  // and the separate pre / post kernels here
+  auto batch_fused = [](const gnufft_plan q) { return q->fused && fused_fft_supported(q->g); };
   b->dim = g0.dim;
   b->nx = nx;
   b->fused = false;
-  for (uint32_t i = 0; i < n_plans; i++) b->fused = b->fused || batch_fused(plans[i]);
+  b->fft_prime = false;
+  for (uint32_t i = 0; i < n_plans; i++) {
+    b->fused = b->fused || batch_fused(plans[i]);
+    b->fft_prime = b->fft_prime || (batch_fused(plans[i]) && fused_fft_prime(plans[i]->g));
+  }
   // batch order: grouped by grid size (one rocFFT plan per group)
   std::vector<uint32_t> order(n_plans);
   for (uint32_t i = 0; i < n_plans; i++) order[i] = i;
@@ -1538,9 +1543,9 @@ int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* 
   HIP_TRY(launch_fold_batch(b->dim, b->ops_d, b->fold, b->fold_threads, s));
   if (b->fused) {
     for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++) {
-      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, false, b->fft_desc_d, b->fft_adj[ps].items,
-                                     b->fft_adj[ps].n, b->fft_adj[ps].lds, s));
-      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, true, b->fft_desc_d,
+      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, false, b->fft_prime, b->fft_desc_d,
+                                     b->fft_adj[ps].items, b->fft_adj[ps].n, b->fft_adj[ps].lds, s));
+      HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, true, b->fft_prime, b->fft_desc_d,
                                      b->fft_adj_big[ps].items, b->fft_adj_big[ps].n,
                                      b->fft_adj_big[ps].lds, s));
     }
@@ -1576,9 +1581,9 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
   }
   if (b->fused) {
     for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++) {
-      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, false, b->fft_desc_d, b->fft_fwd[ps].items,
-                                     b->fft_fwd[ps].n, b->fft_fwd[ps].lds, s));
-      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, true, b->fft_desc_d,
+      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, false, b->fft_prime, b->fft_desc_d,
+                                     b->fft_fwd[ps].items, b->fft_fwd[ps].n, b->fft_fwd[ps].lds, s));
+      HIP_TRY(launch_fused_fft_batch(b->dim, false, ps, true, b->fft_prime, b->fft_desc_d,
                                      b->fft_fwd_big[ps].items, b->fft_fwd_big[ps].n,
                                      b->fft_fwd_big[ps].lds, s));
     }

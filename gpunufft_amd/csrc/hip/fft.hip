@@ -717,6 +717,7 @@ struct FftOpDesc {
   int n_ax;            // d - 1
 };
 
+template <bool PR>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_batch_kernel(const FftOpDesc* __restrict__ d,
                                                                         const uint2* __restrict__ items,
                                                                         int adjoint, int which) {
@@ -725,29 +726,32 @@ __global__ void __launch_bounds__(kFftThreads, 4) fft_axis_batch_kernel(const Ff
   // register copies: the descriptors live in global memory, which the
   // passes also store to
   const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
-  fft_axis_body<kFftThreads, false>(p, (int)it.y);
+  fft_axis_body<kFftThreads, PR>(p, (int)it.y);
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kBigThreads, 1) fft_axis_batch_big_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items, int adjoint, int which) {
   const uint2 it = items[blockIdx.x];
   const FftOpDesc& o = d[it.x];
   const AxisPass p = adjoint ? o.adj_ax[which] : o.fwd_ax[which];
-  fft_axis_body<kBigThreads, false>(p, (int)it.y);
+  fft_axis_body<kBigThreads, PR>(p, (int)it.y);
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_adjoint_last_batch_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const LastPass p = d[it.x].adj_x;
-  fft_adjoint_last_body(p, (int)it.y, RunTable<false>{p.st, p.n_st});
+  fft_adjoint_last_body(p, (int)it.y, RunTable<PR>{p.st, p.n_st});
 }
 
+template <bool PR>
 __global__ void __launch_bounds__(kFftThreads, 4) fft_forward_first_batch_kernel(
     const FftOpDesc* __restrict__ d, const uint2* __restrict__ items) {
   const uint2 it = items[blockIdx.x];
   const FirstPass p = d[it.x].fwd_x;
-  fft_forward_first_body(p, (int)it.y, RunTable<false>{p.st, p.n_st});
+  fft_forward_first_body(p, (int)it.y, RunTable<PR>{p.st, p.n_st});
 }
 
 // ---------------------------------------------------------------------------
@@ -1108,11 +1112,10 @@ bool fused_fft_preferred(const Geom& g) {
   return true;
 }
 
-bool fused_fft_batchable(const Geom& g) {
-  if (!fused_fft_supported(g)) return false;
+bool fused_fft_prime(const Geom& g) {
   for (int a = 0; a < g.dim; a++)
-    if (prime_radix(g.G[a])) return false;
-  return true;
+    if (prime_radix(g.G[a])) return true;
+  return false;
 }
 
 size_t fused_fft_tmp_elems(const Geom& g) {
@@ -1234,35 +1237,43 @@ size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass) {
 }
 
 // One pass of every operator: items (operator, block), descs in device memory.
-hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, bool big, const void* descs_d,
-                                  const uint2* items, uint32_t n_items, size_t lds,
-                                  hipStream_t s) {
-  if (n_items == 0) return hipSuccess;
-  const FftOpDesc* d = static_cast<const FftOpDesc*>(descs_d);
+// PR: the kernel variants with the prime stages 7 .. 19.
+template <bool PR>
+hipError_t launch_fft_batch_pass(int dim, bool adjoint, int pass, bool big, const FftOpDesc* d,
+                                 const uint2* items, uint32_t n_items, size_t lds, hipStream_t s) {
   const int n_ax = dim - 1;
   hipError_t e = hipSuccess;
   if (adjoint && pass == n_ax) {
-    e = allow_lds(fft_adjoint_last_batch_kernel, lds);
+    e = allow_lds(fft_adjoint_last_batch_kernel<PR>, lds);
     if (e == hipSuccess)
-      hipLaunchKernelGGL(fft_adjoint_last_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s,
-                         d, items);
+      hipLaunchKernelGGL(fft_adjoint_last_batch_kernel<PR>, dim3(n_items), dim3(kFftThreads), lds,
+                         s, d, items);
   } else if (!adjoint && pass == 0) {
-    e = allow_lds(fft_forward_first_batch_kernel, lds);
+    e = allow_lds(fft_forward_first_batch_kernel<PR>, lds);
     if (e == hipSuccess)
-      hipLaunchKernelGGL(fft_forward_first_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s,
-                         d, items);
+      hipLaunchKernelGGL(fft_forward_first_batch_kernel<PR>, dim3(n_items), dim3(kFftThreads), lds,
+                         s, d, items);
   } else if (big) {
-    e = allow_lds(fft_axis_batch_big_kernel, lds);
+    e = allow_lds(fft_axis_batch_big_kernel<PR>, lds);
     if (e == hipSuccess)
-      hipLaunchKernelGGL(fft_axis_batch_big_kernel, dim3(n_items), dim3(kBigThreads), lds, s, d,
-                         items, adjoint ? 1 : 0, adjoint ? pass : pass - 1);
+      hipLaunchKernelGGL(fft_axis_batch_big_kernel<PR>, dim3(n_items), dim3(kBigThreads), lds, s,
+                         d, items, adjoint ? 1 : 0, adjoint ? pass : pass - 1);
   } else {
-    e = allow_lds(fft_axis_batch_kernel, lds);
+    e = allow_lds(fft_axis_batch_kernel<PR>, lds);
     if (e == hipSuccess)
-      hipLaunchKernelGGL(fft_axis_batch_kernel, dim3(n_items), dim3(kFftThreads), lds, s, d, items,
-                         adjoint ? 1 : 0, adjoint ? pass : pass - 1);
+      hipLaunchKernelGGL(fft_axis_batch_kernel<PR>, dim3(n_items), dim3(kFftThreads), lds, s, d,
+                         items, adjoint ? 1 : 0, adjoint ? pass : pass - 1);
   }
   return e != hipSuccess ? e : hipGetLastError();
+}
+
+hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, bool big, bool prime,
+                                  const void* descs_d, const uint2* items, uint32_t n_items,
+                                  size_t lds, hipStream_t s) {
+  if (n_items == 0) return hipSuccess;
+  const FftOpDesc* d = static_cast<const FftOpDesc*>(descs_d);
+  if (prime) return launch_fft_batch_pass<true>(dim, adjoint, pass, big, d, items, n_items, lds, s);
+  return launch_fft_batch_pass<false>(dim, adjoint, pass, big, d, items, n_items, lds, s);
 }
 
 }  // namespace gnufft

@@ -325,8 +325,8 @@ hipError_t launch_pre_batch(const BatchOp* ops, const BatchList& l, hipStream_t 
 // Fused oversampled-grid FFT (fft.hip).
 bool fused_fft_supported(const Geom& g);
 bool fused_fft_preferred(const Geom& g);
-// Supported by the batched fused FFT (no radix 7 .. 19 stage).
-bool fused_fft_batchable(const Geom& g);
+// Some axis of g needs a prime stage (7 .. 19).
+bool fused_fft_prime(const Geom& g);
 size_t fused_fft_tmp_elems(const Geom& g);
 std::vector<float2> fused_fft_twiddles(const Geom& g, size_t offsets[3][3]);
 hipError_t fused_fft_adjoint(const Geom& g, float2* grid, float2* tmp, int coils,
@@ -350,8 +350,9 @@ int fused_fft_pass_count(int dim);
 uint32_t fused_fft_pass_blocks(const void* desc, bool adjoint, int pass);
 size_t fused_fft_pass_lds(const void* desc, bool adjoint, int pass);
 int fused_fft_pass_big(const void* desc, bool adjoint, int pass);
-hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, bool big, const void* descs_d,
-                                  const uint2* items, uint32_t n_items, size_t lds,
-                                  hipStream_t s);
+// prime: some operator's lengths need the prime stages 7 .. 19
+hipError_t launch_fused_fft_batch(int dim, bool adjoint, int pass, bool big, bool prime,
+                                  const void* descs_d, const uint2* items, uint32_t n_items,
+                                  size_t lds, hipStream_t s);
 
 }  // namespace gnufft

@@ -117,7 +117,7 @@ int gnufft_plan_get_info(gnufft_plan plan, gnufft_plan_info* info);
  * GNUFFT_FFT_FUSED: one pass per axis, pruned to the image range, with those
  *   stages fused into the first and last pass (grid lengths up to 2048 whose
  *   prime factors are 2, 3, 5, 7, 11, 13, 17 or 19; the default for lengths
- *   up to 1023). Inside a batch, plans with a factor 7 .. 19 use rocFFT.
+ *   up to 1023). A batch runs its plans on their own FFT kind.
  * GNUFFT_FFT_ROCFFT: rocFFT plus the fused pre / post kernels (any length).
  * set_fft returns GNUFFT_ERR_UNSUPPORTED for FUSED on other grid lengths. */
 enum { GNUFFT_FFT_FUSED = 0, GNUFFT_FFT_ROCFFT = 1 };

@@ -4,7 +4,8 @@ oracle (rel-L2 <= 1e-5), for uneven image sizes, trajectory lengths, coil
 counts, with and without sensitivities, in 2-D and 3-D, with the fused FFT
 (one launch per pass for all plans), with rocFFT (one plan per grid size),
 with both in one batch, and with plans whose grid lengths need the fused
-FFT's prime stages (rocFFT inside the batch)."""
+FFT's prime stages, in 2-D and 3-D (the batched FFT kernels' prime-stage
+variants)."""
 
 import numpy as np
 import pytest
@@ -55,8 +56,8 @@ def _ops_3d():
 
 def _ops_2d_primes():
     """Grid lengths with the fused FFT's prime stages (112 = 16 * 7,
-    104 = 8 * 13) beside smooth ones: alone such plans run the fused FFT,
-    inside a batch rocFFT (the batched FFT kernels omit the prime stages)."""
+    104 = 8 * 13) beside smooth ones: the batch runs the prime-stage
+    variants of its FFT kernels for all its fused plans."""
     out = []
     for j, (N, C) in enumerate([(56, 3), (64, 2), (52, 4), (48, 1)]):
         traj = W.spiral_2d(6 + j, 400 + 50 * j, n_turns=N / 10.0)
@@ -65,8 +66,19 @@ def _ops_2d_primes():
     return out
 
 
+def _ops_3d_primes():
+    """3-D grids of 42 = 2 * 3 * 7 and 39 = 3 * 13 points beside 24 (osf 1.5)."""
+    out = []
+    for j, (N, C) in enumerate([(28, 2), (16, 1), (26, 2)]):
+        traj = W.koosh_3d(96 + 32 * j, 40)
+        sens = W.gaussian_sens((N, N, N), C, seed=j) if j != 1 else None
+        out.append(dict(traj=traj, img=(N, N, N), C=C, sens=sens, dens=W.ramp_density(traj, 2), sw=8))
+    return out
+
+
 @pytest.mark.parametrize("fft", ["fused", "rocfft", "mixed"])
-@pytest.mark.parametrize("make", [_ops_2d, _ops_3d, _ops_2d_primes], ids=["2d", "3d", "2d_primes"])
+@pytest.mark.parametrize("make", [_ops_2d, _ops_3d, _ops_2d_primes, _ops_3d_primes],
+                         ids=["2d", "3d", "2d_primes", "3d_primes"])
 def test_batch_equals_single_plans(oracle, env, make, fft):
     torch, capi = env
     specs = make()
