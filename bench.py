@@ -13,9 +13,10 @@ roofline: the adjoint gridding stage (spread_kernel, reduce_chunks_kernel,
 fold_kernel: samples in HBM -> grid in HBM), algorithmic bytes per launch
 8*M*C + 4*d*M + 4*M (density) + 8*Ng*C, over the stage's average time
 measured with HIP events on the launch stream (library stage timer).
-cpu_baseline: the oracle restatement (single thread) running the same
-adjoint + forward (all 8 coils, sensitivities, density) on the host; its
-outputs double as the full-size parity check of the GPU outputs.
+cpu_baseline: the oracle restatement running the same adjoint + forward
+(all 8 coils, sensitivities, density) on the host, OpenMP over all cores
+this process may use (and, beside it, on one thread); its outputs double as
+the full-size parity check of the GPU outputs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -42,11 +43,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the batch and serving figures of the default run")
     ap.add_argument("--workload", default=WORKLOAD,
                     choices=[WORKLOAD, "uneven_batch_2d", "serving"],
                     help="headline 3-D workload, or 64 uneven 2-D operators batched vs "
                          "sequential plain launches")
-    ap.add_argument("--requests", type=int, default=40, help="serving: number of requests")
+    ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=30.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent requests")
     ap.add_argument("--sector-widths", default="4,8,16",
@@ -84,9 +87,11 @@ def main():
 
     torch.cuda.set_device(0)
     if args.workload == "uneven_batch_2d":
-        return bench_batch(args)
+        print(json.dumps(bench_batch(args)))
+        return
     if args.workload == "serving":
-        return bench_serving(args)
+        print(json.dumps(bench_serving(args)))
+        return
     w = W.config(WORKLOAD)
     traj, img, C, dens = w["traj"], w["img_dims"], w["coils"], w["dens"]
     d, M = traj.shape
@@ -193,13 +198,54 @@ def main():
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["parity_rel_l2"] = cpu_baseline(w, plan, sens, kd, torch)
+    del plan, k_dev, img_dev, k_out
+    torch.cuda.empty_cache()
+    if not args.headline_only:
+        # the metric's other two figures, measured in the same run: the
+        # batched-over-plain speed-up of the uneven 2-D batch and the
+        # CG-SENSE serving latencies
+        b = bench_batch(args)
+        out["batch_uneven_2d"] = {k: b[k] for k in (
+            "speedup_batched_vs_plain", "ms_per_step", "plain_ms_per_step", "value",
+            "batched_vs_plain_rel_l2")}
+        out["batch_uneven_2d"]["sector_widths"] = sorted(set(b["config"]["sector_widths"]))
+        sv = bench_serving(args)
+        out["serving_cg_sense"] = {k: sv[k] for k in (
+            "p50_ms", "p99_ms", "mean_ms", "value", "unit", "peak_concurrent", "saturated",
+            "one_at_a_time")}
+        out["serving_cg_sense"]["config"] = sv["config"]["workload"]
     print(json.dumps(out))
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_flags():
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("CXXFLAGS"):
+                    return "g++ " + line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, plan, sens, kd, torch):
-    """Oracle restatement, one thread: the same adjoint + forward (all coils,
-    sensitivities, density) on the host. Its outputs are the full-size parity
-    check of the GPU adjoint and forward on the same inputs."""
+    """The oracle restatement of the same adjoint + forward (all coils,
+    sensitivities, density) on the host cores: OpenMP over every core this
+    process may run on (sectors shared out, one private grid per thread,
+    summed at the end; threaded FFTs), and one thread as the reference's
+    own CPU gridding runs. The multi-core outputs are also the full-size
+    parity check of the GPU adjoint and forward on the same inputs."""
     import numpy as np
 
     from oracle import oracle as O
@@ -207,26 +253,40 @@ def cpu_baseline(w, plan, sens, kd, torch):
     traj, img = w["traj"], w["img_dims"]
     M = traj.shape[1]
     C = kd.shape[0]
-    op = O.Operator(traj, img, w["kw"], w["sw"], w["osf"], dens=w["dens"], sens=sens,
-                    order="stable")
+    cores = len(os.sched_getaffinity(0))
+    op_mt = O.Operator(traj, img, w["kw"], w["sw"], w["osf"], dens=w["dens"], sens=sens,
+                       order="stable", threads=cores)
     t0 = time.perf_counter()
-    x_ref = op.adjoint(kd)
-    k_ref = op.forward(x_ref)
-    t = time.perf_counter() - t0
+    x_ref = op_mt.adjoint(kd)
+    k_ref = op_mt.forward(x_ref)
+    t_mt = time.perf_counter() - t0
+    op_1 = O.Operator(traj, img, w["kw"], w["sw"], w["osf"], dens=w["dens"], sens=sens,
+                      order="stable")
+    t0 = time.perf_counter()
+    x_1 = op_1.adjoint(kd)
+    k_1 = op_1.forward(x_1)
+    t_1 = time.perf_counter() - t0
     x_gpu = torch.empty(x_ref.shape, dtype=torch.complex64, device="cuda")
     plan.adjoint(torch.from_numpy(np.ascontiguousarray(kd)).cuda(), x_gpu, C)
     k_gpu = torch.empty((C, M), dtype=torch.complex64, device="cuda")
     plan.forward(torch.from_numpy(np.ascontiguousarray(x_ref)).cuda(), k_gpu, C)
     torch.cuda.synchronize()
     parity = {"adjoint": O.rel_l2(x_gpu.cpu().numpy(), x_ref),
-              "forward": O.rel_l2(k_gpu.cpu().numpy(), k_ref)}
+              "forward": O.rel_l2(k_gpu.cpu().numpy(), k_ref),
+              "cpu_1_thread_vs_cpu_%d_threads" % cores: max(O.rel_l2(x_1, x_ref),
+                                                              O.rel_l2(k_1, k_ref))}
     return ({
-        "value": round(2.0 * M * C / t, 1),
+        "value": round(2.0 * M * C / t_mt, 1),
         "unit": "gridded samples/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": "full workload: adjoint+forward of all %d coils (M=%d, sens, density), "
-                  "oracle restatement, 1 thread, %.1f s" % (C, M, t),
+        "sample": "full workload: adjoint+forward of all %d coils (M=%d, sens, density), oracle "
+                  "restatement, OpenMP over %d threads (private grids) + scipy.fft workers=%d, "
+                  "%.2f s" % (C, M, cores, cores, t_mt),
+        "cpu_model": _cpu_model(),
+        "compiler_flags": _oracle_flags(),
+        "single_thread": {"value": round(2.0 * M * C / t_1, 1), "cores": 1,
+                          "seconds": round(t_1, 2)},
     }, parity)
 
 
@@ -319,7 +379,10 @@ def bench_batch(args):
         "batched_vs_plain_rel_l2": err,
         "setup_s": round(setup_s, 2),
     }
-    print(json.dumps(out))
+    batch.close()
+    for p in plans:
+        p.close()
+    return out
 
 
 def bench_serving(args):
@@ -341,6 +404,12 @@ def bench_serving(args):
     cont = serve.Server(slots=args.slots).run(reqs)
     seq = serve.Server(slots=1).run(serve.serving_mix(n_requests=args.requests,
                                                       rate_hz=args.rate, seed=7))
+    # saturated: every request waiting at t = 0, so the server, not the
+    # arrival process, sets the rate (the mix's capacity in requests/s)
+    sat_reqs = serve.serving_mix(n_requests=args.requests, rate_hz=args.rate, seed=7)
+    for r in sat_reqs:
+        r.arrival = 0.0
+    sat = serve.Server(slots=args.slots).run(sat_reqs)
     n2 = sum(1 for r in reqs if len(r.img_dims) == 2)
     out = {
         "metric": "CG-SENSE serving: p50/p99 latency (ms) and requests/s, continuous batching",
@@ -365,8 +434,10 @@ def bench_serving(args):
         "mean_ms": round(cont["mean_ms"], 3),
         "peak_concurrent": cont["peak_concurrent"],
         "one_at_a_time": {k: round(v, 3) if isinstance(v, float) else v for k, v in seq.items()},
+        "saturated": {"requests_per_s": round(sat["requests_per_s"], 3),
+                      "p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3)},
     }
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

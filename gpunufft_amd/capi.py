@@ -32,7 +32,7 @@ EXPORTS = [
     "gnufft_spread_sorted", "gnufft_interp_sorted", "gnufft_plan_set_timing",
     "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
     "gnufft_batch_adjoint", "gnufft_batch_forward", "gnufft_cg_create", "gnufft_cg_destroy",
-    "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft",
+    "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft", "gnufft_check_precomputed",
 ]
 
 T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
@@ -108,6 +108,7 @@ def load(path=LIB_PATH):
     L.gnufft_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
     L.gnufft_plan_set_trajectory.argtypes = [vp, vp, u32, vp, i32, vp]
     L.gnufft_plan_load_precomputed.argtypes = [vp, vp, vp, vp, vp, vp, vp, u32, i32, vp]
+    L.gnufft_check_precomputed.argtypes = [u32p, u32, f32, f32p, u32p, u32p, u32p, u32]
     L.gnufft_plan_set_sens.argtypes = [vp, vp, u32, i32, vp]
     L.gnufft_plan_get_tables.argtypes = [vp, u32p, u32p, u32p, f32p, f32p, f32p, f32p]
     L.gnufft_plan_get_assigned_sectors.argtypes = [vp, u32p]
@@ -157,6 +158,19 @@ def _ptr_loc(a):
         return ctypes.c_void_p(a.ctypes.data), HOST
     assert a.is_contiguous(), "device tensors must be contiguous"
     return ctypes.c_void_p(a.data_ptr()), (DEVICE if a.is_cuda else HOST)
+
+
+def check_precomputed(img_dims, sector_width, osf, traj_sorted, data_indices,
+                      sector_data_count, sector_centers):
+    """Host-only table check (gnufft_check_precomputed); returns the status code."""
+    dims = (list(img_dims) + [0, 0, 0])[:3]
+    tr = np.ascontiguousarray(traj_sorted, np.float32)
+    di = np.ascontiguousarray(data_indices, np.uint32)
+    sdc = np.ascontiguousarray(sector_data_count, np.uint32)
+    cen = np.ascontiguousarray(sector_centers, np.uint32)
+    return load().gnufft_check_precomputed((ctypes.c_uint32 * 3)(*dims), sector_width, float(osf),
+                                           _f32p(tr), _u32p(di), _u32p(sdc), _u32p(cen),
+                                           tr.shape[-1])
 
 
 def _u32p(a):

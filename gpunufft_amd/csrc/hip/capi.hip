@@ -266,6 +266,12 @@ struct gnufft_plan_s {
   float2* fft_tw_d = nullptr;
   size_t fft_tw_off[3][3] = {};
   StageTimer timer;
+  // bumped whenever device memory that a captured graph or a batch may
+  // hold a pointer to is freed or replaced (trajectory tables, work grid,
+  // scratch, sensitivities, image buffer) or the FFT kind changes
+  uint64_t generation = 0;
+  // the part of that a batch depends on: trajectory tables and sensitivities
+  uint64_t table_generation = 0;
 };
 
 // Runs `expr` (a hipError_t launch) bracketed by the plan's stage timer.
@@ -277,6 +283,8 @@ struct gnufft_plan_s {
     HIP_TRY(t_e_);                                             \
   } while (0)
 
+uint64_t gnufft::plan_generation(const gnufft_plan_s* p) { return p ? p->generation : 0; }
+
 namespace {
 
 void dfree(void* p) {
@@ -284,6 +292,8 @@ void dfree(void* p) {
 }
 
 void free_traj(gnufft_plan p) {
+  p->generation++;
+  p->table_generation++;
   dfree(p->traj_sorted_d);
   dfree(p->dens_sorted_d);
   dfree(p->idx_d);
@@ -411,6 +421,7 @@ int compute_deapo(gnufft_plan p, hipStream_t s) {
 
 int ensure_grid(gnufft_plan p, uint32_t coils) {
   if (p->grid_coils >= coils) return GNUFFT_OK;
+  p->generation++;
   dfree(p->grid_d);
   p->grid_d = nullptr;
   p->grid_coils = 0;
@@ -436,6 +447,7 @@ size_t scratch_per_coil(gnufft_plan p) {
 
 int ensure_scratch(gnufft_plan p, uint32_t coils) {
   if (p->scratch_coils >= coils) return GNUFFT_OK;
+  p->generation++;
   dfree(p->scratch_d);
   p->scratch_d = nullptr;
   p->scratch_coils = 0;
@@ -462,6 +474,8 @@ uint32_t coil_batch(gnufft_plan p, uint32_t n_coils) {
 }
 
 // In-place C2C rocFFT plans (both directions) of `batch` grids of g's size.
+// On any failure everything made so far is released (plans, whether new or
+// taken from the pool, execution info and work buffer).
 int create_fft(const Geom& g, uint32_t batch, FftPlans* out) {
   FftPlans f;
   size_t lengths[3] = {(size_t)g.G[0], (size_t)g.G[1], (size_t)g.G[2]};
@@ -472,21 +486,42 @@ int create_fft(const Geom& g, uint32_t batch, FftPlans* out) {
   f.key.G1 = g.G[1];
   f.key.G2 = g.G[2];
   f.key.batch = batch;
+  auto bail = [&](int code, const std::string& msg) {
+    if (f.fwd) rocfft_plan_destroy(f.fwd);
+    if (f.inv) rocfft_plan_destroy(f.inv);
+    if (f.info) rocfft_execution_info_destroy(f.info);
+    if (f.work) (void)hipFree(f.work);
+    f = FftPlans();
+    return fail(code, msg);
+  };
   if (!take_pooled_fft(f.key, &f.fwd, &f.inv)) {
-    ROCFFT_TRY(rocfft_plan_create(&f.fwd, rocfft_placement_inplace,
-                                  rocfft_transform_type_complex_forward, rocfft_precision_single,
-                                  dims, lengths, batch, nullptr));
-    ROCFFT_TRY(rocfft_plan_create(&f.inv, rocfft_placement_inplace,
-                                  rocfft_transform_type_complex_inverse, rocfft_precision_single,
-                                  dims, lengths, batch, nullptr));
+    if (rocfft_plan_create(&f.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                           rocfft_precision_single, dims, lengths, batch, nullptr) != rocfft_status_success) {
+      f.fwd = nullptr;
+      return bail(GNUFFT_ERR_FFT, "rocfft_plan_create (forward) failed");
+    }
+    if (rocfft_plan_create(&f.inv, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                           rocfft_precision_single, dims, lengths, batch, nullptr) != rocfft_status_success) {
+      f.inv = nullptr;
+      return bail(GNUFFT_ERR_FFT, "rocfft_plan_create (inverse) failed");
+    }
   }
   size_t w1 = 0, w2 = 0;
-  ROCFFT_TRY(rocfft_plan_get_work_buffer_size(f.fwd, &w1));
-  ROCFFT_TRY(rocfft_plan_get_work_buffer_size(f.inv, &w2));
+  if (rocfft_plan_get_work_buffer_size(f.fwd, &w1) != rocfft_status_success ||
+      rocfft_plan_get_work_buffer_size(f.inv, &w2) != rocfft_status_success)
+    return bail(GNUFFT_ERR_FFT, "rocfft_plan_get_work_buffer_size failed");
   f.work_bytes = std::max(w1, w2);
-  if (f.work_bytes) HIP_TRY(hipMalloc(&f.work, f.work_bytes));
-  ROCFFT_TRY(rocfft_execution_info_create(&f.info));
-  if (f.work_bytes) ROCFFT_TRY(rocfft_execution_info_set_work_buffer(f.info, f.work, f.work_bytes));
+  if (f.work_bytes && hipMalloc(&f.work, f.work_bytes) != hipSuccess) {
+    f.work = nullptr;
+    return bail(GNUFFT_ERR_OUT_OF_MEMORY, "rocFFT work buffer allocation");
+  }
+  if (rocfft_execution_info_create(&f.info) != rocfft_status_success) {
+    f.info = nullptr;
+    return bail(GNUFFT_ERR_FFT, "rocfft_execution_info_create failed");
+  }
+  if (f.work_bytes &&
+      rocfft_execution_info_set_work_buffer(f.info, f.work, f.work_bytes) != rocfft_status_success)
+    return bail(GNUFFT_ERR_FFT, "rocfft_execution_info_set_work_buffer failed");
   *out = f;
   return GNUFFT_OK;
 }
@@ -786,6 +821,7 @@ int gnufft_plan_create(gnufft_plan* plan, const uint32_t img_dims[3], uint32_t k
 
 int gnufft_plan_set_fft(gnufft_plan p, int kind) {
   if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_fft: null plan");
+  p->generation++;
   if (kind == GNUFFT_FFT_ROCFFT) {
     p->fused = false;
     return GNUFFT_OK;
@@ -930,6 +966,73 @@ done:
   return rc;
 }
 
+int gnufft_check_precomputed(const uint32_t img_dims[3], uint32_t sector_width, float osf,
+                             const float* traj_sorted, const uint32_t* data_indices,
+                             const uint32_t* sector_data_count, const uint32_t* sector_centers,
+                             uint32_t n_samples) {
+  if (!img_dims || !sector_data_count || !sector_centers || ((!traj_sorted || !data_indices) && n_samples))
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "check_precomputed: null argument");
+  if (sector_width == 0 || !(osf >= 1.0f) || img_dims[0] == 0 || img_dims[1] == 0)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "check_precomputed: bad geometry");
+  const int D = img_dims[2] == 0 ? 2 : 3;
+  const uint32_t M = n_samples;
+  int G[3] = {1, 1, 1};
+  uint32_t S[3] = {1, 1, 1};
+  for (int a = 0; a < D; a++) {
+    G[a] = (int)(uint32_t)(img_dims[a] * osf);  // gpuNUFFT_types.hpp:109-117
+    if (G[a] <= 0) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "check_precomputed: empty grid");
+    S[a] = (uint32_t)std::ceil((float)G[a] / (float)sector_width);
+  }
+  const uint64_t nS = (uint64_t)S[0] * S[1] * S[2];
+  // sector offsets: length S+1, starting at 0, non-decreasing, ending at M
+  // (computeSectorDataCount, gpuNUFFT_operator_factory.cpp:183-205)
+  if (sector_data_count[0] != 0)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "check_precomputed: sector_data_count[0] != 0");
+  for (uint64_t s = 0; s < nS; s++)
+    if (sector_data_count[s + 1] < sector_data_count[s])
+      return fail(GNUFFT_ERR_INVALID_ARGUMENT,
+                  "check_precomputed: sector_data_count decreases at sector " + std::to_string(s));
+  if (sector_data_count[nS] != M)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "check_precomputed: sector_data_count[S] != sample count");
+  // data indices: a permutation of [0, M)
+  {
+    std::vector<uint8_t> seen(M, 0);
+    for (uint32_t i = 0; i < M; i++) {
+      const uint32_t v = data_indices[i];
+      if (v >= M || seen[v])
+        return fail(GNUFFT_ERR_INVALID_ARGUMENT,
+                    "check_precomputed: data_indices is not a permutation of [0, M) (entry " +
+                        std::to_string(i) + ")");
+      seen[v] = 1;
+    }
+  }
+  // centers in the standard layout (gpuNUFFT_operator_factory.cpp:207-270):
+  // the kernels derive padded-sector positions from it
+  for (uint64_t sct = 0; sct < nS; sct++) {
+    const uint32_t idx[3] = {(uint32_t)(sct % S[0]), (uint32_t)((sct / S[0]) % S[1]),
+                             (uint32_t)(sct / ((uint64_t)S[0] * S[1]))};
+    for (int a = 0; a < D; a++)
+      if (sector_centers[sct * D + a] != idx[a] * sector_width + sector_width / 2)
+        return fail(GNUFFT_ERR_INVALID_ARGUMENT,
+                    "check_precomputed: sector centers differ from the plan's sector layout");
+  }
+  // every sorted sample lies in the sector its range names (E3 assignment,
+  // CUDA/inc/precomp_utils.hpp:34-48, :142-153)
+  for (uint64_t sct = 0; sct < nS; sct++)
+    for (uint32_t i = sector_data_count[sct]; i < sector_data_count[sct + 1]; i++) {
+      uint64_t lin = 0, mul = 1;
+      for (int a = 0; a < D; a++) {
+        lin += mul * (uint64_t)host_sector_of(traj_sorted[(size_t)a * M + i], G[a], (int)sector_width);
+        mul *= S[a];
+      }
+      if (lin != sct)
+        return fail(GNUFFT_ERR_INVALID_ARGUMENT,
+                    "check_precomputed: sorted sample " + std::to_string(i) + " lies outside sector " +
+                        std::to_string(sct));
+    }
+  return GNUFFT_OK;
+}
+
 int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
                                  const uint32_t* data_indices, const uint32_t* sector_data_count,
                                  const uint32_t* sector_centers, const float* dens_sorted,
@@ -938,9 +1041,36 @@ int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
   if (!p || !sector_data_count || !sector_centers || ((!traj_sorted || !data_indices) && n_samples))
     return fail(GNUFFT_ERR_INVALID_ARGUMENT, "load_precomputed: null argument");
   hipStream_t s = (hipStream_t)stream;
-  free_traj(p);
   const uint32_t M = n_samples;
   const int D = p->g.dim;
+  {
+    // validate on the host before anything reaches the kernels: a bad index
+    // or offset table would otherwise become out-of-range device accesses
+    std::vector<float> tr;
+    std::vector<uint32_t> di, sdc, cen;
+    const float* tr_h = traj_sorted;
+    const uint32_t *di_h = data_indices, *sdc_h = sector_data_count, *cen_h = sector_centers;
+    if (location == GNUFFT_DEVICE) {
+      tr.resize((size_t)D * M);
+      di.resize(M);
+      sdc.resize((size_t)p->S + 1);
+      cen.resize((size_t)p->S * D);
+      if (M) {
+        HIP_TRY(hipMemcpy(tr.data(), traj_sorted, tr.size() * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(di.data(), data_indices, di.size() * 4, hipMemcpyDeviceToHost));
+      }
+      HIP_TRY(hipMemcpy(sdc.data(), sector_data_count, sdc.size() * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(cen.data(), sector_centers, cen.size() * 4, hipMemcpyDeviceToHost));
+      tr_h = tr.data();
+      di_h = di.data();
+      sdc_h = sdc.data();
+      cen_h = cen.data();
+    }
+    const uint32_t dims[3] = {p->img[0], p->img[1], D == 3 ? p->img[2] : 0u};
+    int rc = gnufft_check_precomputed(dims, p->sw, p->osf, tr_h, di_h, sdc_h, cen_h, M);
+    if (rc) return rc;
+  }
+  free_traj(p);
   int rc = copy_in((void**)&p->traj_sorted_d, traj_sorted, (size_t)D * M * 4, location, s);
   if (!rc) rc = copy_in((void**)&p->idx_d, data_indices, (size_t)M * 4, location, s);
   if (!rc) rc = copy_in((void**)&p->sdc_d, sector_data_count, (size_t)(p->S + 1) * 4, location, s);
@@ -950,25 +1080,8 @@ int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
     HIP_TRY(hipMemcpyAsync(p->deapo_d, deapo, (size_t)p->g.Ni * 4,
                            location == GNUFFT_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                            s));
-  if (!rc) {
-    HIP_TRY(hipStreamSynchronize(s));
-    // The gridding kernels derive padded-sector positions from the sector
-    // layout (center = index * sw + floor(sw / 2),
-    // gpuNUFFT_operator_factory.cpp:207-214);
-    // tables from another layout would be gridded to the wrong place.
-    std::vector<uint32_t> c((size_t)p->S * D);
-    HIP_TRY(hipMemcpy(c.data(), p->centers_d, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    for (uint32_t sct = 0; sct < p->S && !rc; sct++) {
-      const uint32_t idx[3] = {sct % p->sectors[0], (sct / p->sectors[0]) % p->sectors[1],
-                               sct / (p->sectors[0] * p->sectors[1])};
-      for (int a = 0; a < D; a++)
-        if (c[(size_t)sct * D + a] != idx[a] * p->sw + p->sw / 2) {
-          rc = fail(GNUFFT_ERR_INVALID_ARGUMENT,
-                    "load_precomputed: sector centers differ from the plan's sector layout");
-          break;
-        }
-    }
-  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(GNUFFT_ERR_DEVICE, "load_precomputed: table upload");
   if (!rc) {
     p->M = M;
     p->g.M = (int)M;
@@ -981,6 +1094,8 @@ int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
 int gnufft_plan_set_sens(gnufft_plan p, const void* sens, uint32_t n_coils, int location,
                          void* stream) {
   if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_sens: null plan");
+  p->generation++;
+  p->table_generation++;
   dfree(p->sens_d);
   p->sens_d = nullptr;
   p->n_sens = 0;
@@ -1001,13 +1116,54 @@ int gnufft_plan_get_tables(gnufft_plan p, uint32_t* data_indices, uint32_t* sect
   if ((data_indices || sector_data_count || sector_centers || traj_sorted || dens_sorted) &&
       !p->sdc_d)
     return fail(GNUFFT_ERR_STATE, "get_tables: no trajectory set");
-  if (data_indices && p->M) HIP_TRY(hipMemcpy(data_indices, p->idx_d, (size_t)p->M * 4, hipMemcpyDeviceToHost));
+  if (dens_sorted && !p->dens_sorted_d) return fail(GNUFFT_ERR_STATE, "get_tables: no density compensation");
+  const uint32_t M = p->M;
   if (sector_data_count) HIP_TRY(hipMemcpy(sector_data_count, p->sdc_d, (size_t)(p->S + 1) * 4, hipMemcpyDeviceToHost));
   if (sector_centers) HIP_TRY(hipMemcpy(sector_centers, p->centers_d, (size_t)p->S * D * 4, hipMemcpyDeviceToHost));
-  if (traj_sorted && p->M) HIP_TRY(hipMemcpy(traj_sorted, p->traj_sorted_d, (size_t)p->M * D * 4, hipMemcpyDeviceToHost));
-  if (dens_sorted) {
-    if (!p->dens_sorted_d) return fail(GNUFFT_ERR_STATE, "get_tables: no density compensation");
-    HIP_TRY(hipMemcpy(dens_sorted, p->dens_sorted_d, (size_t)p->M * 4, hipMemcpyDeviceToHost));
+  if ((data_indices || traj_sorted || dens_sorted) && M) {
+    // The kernels run on the device sort's order (stable within a sector).
+    // A computed plan reports the reference's order instead: std::sort of
+    // (index, sector) pairs compared by sector only, which libstdc++'s
+    // introsort leaves in an implementation-defined order within sectors
+    // (CUDA/src/gpuNUFFT_operator_factory.cpp:56-72, comparator
+    // CUDA/inc/gpuNUFFT_types.hpp:33-49; tables gathered as at :499-522).
+    // Loaded plans report the tables they were given.
+    std::vector<uint32_t> idx(M);
+    HIP_TRY(hipMemcpy(idx.data(), p->idx_d, (size_t)M * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> ref;  // reference position -> device-order position
+    if (p->assigned_d) {
+      std::vector<uint32_t> sec(M);
+      HIP_TRY(hipMemcpy(sec.data(), p->assigned_d, (size_t)M * 4, hipMemcpyDeviceToHost));
+      std::vector<std::pair<uint32_t, uint32_t>> v;
+      v.reserve(M);
+      for (uint32_t i = 0; i < M; i++) v.emplace_back(i, sec[i]);
+      std::sort(v.begin(), v.end(), [](const std::pair<uint32_t, uint32_t>& a,
+                                       const std::pair<uint32_t, uint32_t>& b) {
+        return a.second < b.second;
+      });
+      std::vector<uint32_t> pos(M);  // caller index -> device-order position
+      for (uint32_t i = 0; i < M; i++) pos[idx[i]] = i;
+      ref.resize(M);
+      for (uint32_t i = 0; i < M; i++) ref[i] = pos[v[i].first];
+    }
+    auto reorder = [&](const float* dev, uint32_t rows, float* out) -> int {
+      std::vector<float> h((size_t)rows * M);
+      HIP_TRY(hipMemcpy(h.data(), dev, h.size() * 4, hipMemcpyDeviceToHost));
+      for (uint32_t r = 0; r < rows; r++)
+        for (uint32_t i = 0; i < M; i++)
+          out[(size_t)r * M + i] = h[(size_t)r * M + (ref.empty() ? i : ref[i])];
+      return GNUFFT_OK;
+    };
+    if (data_indices)
+      for (uint32_t i = 0; i < M; i++) data_indices[i] = idx[ref.empty() ? i : ref[i]];
+    if (traj_sorted) {
+      int rc = reorder(p->traj_sorted_d, (uint32_t)D, traj_sorted);
+      if (rc) return rc;
+    }
+    if (dens_sorted) {
+      int rc = reorder(p->dens_sorted_d, 1, dens_sorted);
+      if (rc) return rc;
+    }
   }
   if (deapo) HIP_TRY(hipMemcpy(deapo, p->deapo_d, (size_t)p->g.Ni * 4, hipMemcpyDeviceToHost));
   if (lut) memcpy(lut, p->lut.data(), p->lut.size() * sizeof(float));
@@ -1124,6 +1280,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     if (g.Ng != g.Ni)
       return fail(GNUFFT_ERR_UNSUPPORTED, "forward DENSITY_ESTIMATION needs grid == image size (osf 1)");
     if (p->img_tmp_coils < n_coils) {
+      p->generation++;
       dfree(p->img_tmp_d);
       p->img_tmp_d = nullptr;
       p->img_tmp_coils = 0;
@@ -1193,6 +1350,7 @@ struct gnufft_batch_s {
   std::vector<gnufft_plan> plans;  // in batch order (grouped by grid size)
   std::vector<uint32_t> coils;
   std::vector<uint32_t> caller_index;  // batch position -> caller's plan index
+  std::vector<uint64_t> table_gen;     // each plan's table generation at creation
   std::vector<BatchOp> ops_h;
   BatchOp* ops_d = nullptr;
   uint4* items_d = nullptr;
@@ -1225,12 +1383,14 @@ struct gnufft_batch_s {
   FftPass fft_adj[3], fft_fwd[3];          // passes on the standard tiles
   FftPass fft_adj_big[3], fft_fwd_big[3];  // y / z passes of long lines (large tiles)
   hipEvent_t uploaded = nullptr;  // last descriptor upload (ops_h may be reused after it)
+  hipEvent_t done = nullptr;      // end of the last call's device work, on its stream
 };
 
 namespace {
 
 void free_batch(gnufft_batch b) {
   if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+  if (b->done) (void)hipEventDestroy(b->done);
   dfree(b->ops_d);
   dfree(b->items_d);
   dfree(b->grid_d);
@@ -1241,9 +1401,30 @@ void free_batch(gnufft_batch b) {
   delete b;
 }
 
-// Wait until the previous descriptor upload has read ops_h.
-int batch_wait_upload(gnufft_batch b) {
+// The batch holds its plans' trajectory tables and sensitivities by
+// pointer: a plan whose tables were replaced since batch creation makes
+// the batch unusable.
+int batch_check_plans(gnufft_batch b) {
+  for (size_t k = 0; k < b->plans.size(); k++)
+    if (b->plans[k]->table_generation != b->table_gen[k])
+      return fail(GNUFFT_ERR_STATE,
+                  "batch: plan " + std::to_string(b->caller_index[k]) +
+                      " changed its trajectory or sensitivities after batch_create; create the batch again");
+  return GNUFFT_OK;
+}
+
+// Wait until the previous descriptor upload has read ops_h, and order this
+// call's device work after the previous call's (its descriptors, work grid
+// and scratch are shared), whichever stream that one ran on.
+int batch_wait_upload(gnufft_batch b, hipStream_t s) {
   if (b->uploaded) HIP_TRY(hipEventSynchronize(b->uploaded));
+  if (b->done) HIP_TRY(hipStreamWaitEvent(s, b->done, 0));
+  return GNUFFT_OK;
+}
+
+int batch_mark_done(gnufft_batch b, hipStream_t s) {
+  if (!b->done) HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(b->done, s));
   return GNUFFT_OK;
 }
 
@@ -1324,6 +1505,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     b->plans.push_back(p);
     b->coils.push_back(C);
     b->caller_index.push_back(order[k]);
+    b->table_gen.push_back(p->table_generation);
     b->op_fused.push_back(batch_fused(p) ? 1 : 0);
     grid_off[k] = grid_total;
     grid_total += (size_t)C * p->g.Ng;
@@ -1525,7 +1707,8 @@ int gnufft_batch_destroy(gnufft_batch batch) {
 int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* out, void* stream) {
   if (!b || !kdata || !out) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_adjoint: null argument");
   hipStream_t s = (hipStream_t)stream;
-  int rc0 = batch_wait_upload(b);
+  int rc0 = batch_check_plans(b);
+  if (!rc0) rc0 = batch_wait_upload(b, s);
   if (rc0) return rc0;
   for (size_t k = 0; k < b->ops_h.size(); k++) {
     const uint32_t ci = b->caller_index[k];
@@ -1555,13 +1738,14 @@ int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* 
     if (rc) return rc;
   }
   HIP_TRY(launch_post_batch(b->ops_d, b->post, s));
-  return GNUFFT_OK;
+  return batch_mark_done(b, s);
 }
 
 int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* out, void* stream) {
   if (!b || !img || !out) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "batch_forward: null argument");
   hipStream_t s = (hipStream_t)stream;
-  int rc0 = batch_wait_upload(b);
+  int rc0 = batch_check_plans(b);
+  if (!rc0) rc0 = batch_wait_upload(b, s);
   if (rc0) return rc0;
   for (size_t k = 0; k < b->ops_h.size(); k++) {
     const uint32_t ci = b->caller_index[k];
@@ -1589,7 +1773,7 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
     }
   }
   HIP_TRY(launch_interp_batch(b->dim, b->nx, b->ops_d, b->interp, b->interp_lds, s));
-  return GNUFFT_OK;
+  return batch_mark_done(b, s);
 }
 
 }  // extern "C"

@@ -161,6 +161,7 @@ struct gnufft_cg_s {
   int blocks = kCgBlocks;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t generation = 0;  // the plan's buffer generation the graph was captured at
   bool warm = false;
   // the solver's own stream (default priority, non-blocking: the legacy
   // null stream cannot be captured), ordered after / before the caller's
@@ -250,6 +251,15 @@ int gnufft_cg_destroy(gnufft_cg cg) {
 int gnufft_cg_solve(gnufft_cg c, const void* kdata, void* x_out, uint32_t iterations, int use_graph,
                     void* stream) {
   if (!c || !kdata || !x_out) return cg_fail(GNUFFT_ERR_INVALID_ARGUMENT, "cg_solve: null argument");
+  {
+    // the solver's vectors are sized for the plan's sample and coil counts
+    gnufft_plan_info info;
+    CG_ABI(gnufft_plan_get_info(c->plan, &info));
+    if (info.n_samples != c->samples || info.n_sens_coils != c->coils)
+      return cg_fail(GNUFFT_ERR_STATE,
+                     "cg_solve: the plan's trajectory length or sensitivity coils changed since "
+                     "cg_create; create a new solver");
+  }
   // the work runs on the solver's stream, after the caller's stream's
   // earlier work and before its later work
   const hipStream_t caller = (hipStream_t)stream;
@@ -262,12 +272,25 @@ int gnufft_cg_solve(gnufft_cg c, const void* kdata, void* x_out, uint32_t iterat
                      c->y, c->n, c->part);
   hipLaunchKernelGGL(cg_reduce_kernel, dim3(1), dim3(kCgThreads), 0, s, c->part, c->blocks, c->s, 0);
   CG_HIP(hipGetLastError());
+  // The captured iteration holds the plan's device pointers (tables, work
+  // grid, scratch, sensitivities). When any of them was freed or replaced
+  // since the capture (set_trajectory, set_sens, a larger coil batch,
+  // set_fft), the graph is dropped and captured again.
+  if (use_graph && c->exec && c->generation != gnufft::plan_generation(c->plan)) {
+    CG_HIP(hipStreamSynchronize(s));  // earlier replays may still run
+    (void)hipGraphExecDestroy(c->exec);
+    (void)hipGraphDestroy(c->graph);
+    c->exec = nullptr;
+    c->graph = nullptr;
+    c->warm = false;
+  }
   if (use_graph && !c->exec) {
     if (!c->warm) {
       // allocate the plan's work buffers and FFT plans outside the capture
       CG_ABI(gnufft_forward(c->plan, c->p, c->k, c->coils, GNUFFT_STOP_DEAPODIZATION, s));
       c->warm = true;
     }
+    const uint64_t gen = gnufft::plan_generation(c->plan);
     CG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = cg_iteration(c, s);
     hipGraph_t g = nullptr;
@@ -277,8 +300,14 @@ int gnufft_cg_solve(gnufft_cg c, const void* kdata, void* x_out, uint32_t iterat
       return rc;
     }
     CG_HIP(e);
+    if (gen != gnufft::plan_generation(c->plan)) {
+      // a buffer was replaced while capturing: the graph may hold a freed pointer
+      (void)hipGraphDestroy(g);
+      return cg_fail(GNUFFT_ERR_STATE, "cg_solve: plan buffers changed during graph capture");
+    }
     c->graph = g;
     CG_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+    c->generation = gen;
   }
   for (uint32_t it = 0; it < iterations; it++) {
     if (use_graph) {

@@ -145,6 +145,16 @@ __device__ __host__ inline int wrap(int p, int G) {
 // Message returned by gnufft_last_error() on this thread (capi.hip).
 void set_last_error(const std::string& msg);
 
+}  // namespace gnufft
+
+struct gnufft_plan_s;
+
+namespace gnufft {
+
+// Generation of a plan's device buffers (capi.hip): changes whenever memory
+// that a captured graph or a batch may reference is freed or replaced.
+uint64_t plan_generation(const gnufft_plan_s* p);
+
 // ---------------------------------------------------------------------------
 // Launchers implemented in the .hip translation units (all asynchronous on
 // `stream`; return hipError_t).

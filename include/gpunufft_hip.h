@@ -127,19 +127,36 @@ int gnufft_plan_get_fft(gnufft_plan plan, int* kind);
 /* Sector precompute on the device: assignSectors + sort + sectorDataCount +
  * sorted trajectory/density (CUDA/src/gpuNUFFT_operator_factory.cpp:128-205,
  * :450-553; CUDA/src/gpu/precomp_kernels.cu:8-177). traj/dens live at
- * `location`; dens may be NULL. Samples are ordered by (sector, index):
- * the stable order of the reference's std::sort. */
+ * `location`; dens may be NULL. The kernels order samples by (sector,
+ * index); gnufft_plan_get_tables reports the reference's std::sort order. */
 int gnufft_plan_set_trajectory(gnufft_plan plan, const float* traj, uint32_t n_samples,
                                const float* dens, int location, void* stream);
 
 /* Load precomputed tables (loadPrecomputedGpuNUFFTOperator,
  * CUDA/src/gpuNUFFT_operator_factory.cpp:576-626). traj_sorted is SoA in
- * sector order; dens_sorted and deapo may be NULL (deapo NULL: computed). */
+ * sector order, samples in any order within a sector (the reference's
+ * std::sort order or the stable one); dens_sorted and deapo may be NULL
+ * (deapo NULL: computed). The tables are checked on the host first
+ * (gnufft_check_precomputed); malformed tables return
+ * GNUFFT_ERR_INVALID_ARGUMENT and leave the plan's previous tables freed. */
 int gnufft_plan_load_precomputed(gnufft_plan plan, const float* traj_sorted,
                                  const uint32_t* data_indices, const uint32_t* sector_data_count,
                                  const uint32_t* sector_centers, const float* dens_sorted,
                                  const float* deapo, uint32_t n_samples, int location,
                                  void* stream);
+
+/* Host-only consistency check of precomputed tables (host pointers; needs
+ * no GPU). The reference loads such tables unchecked
+ * (CUDA/src/gpuNUFFT_operator_factory.cpp:576-626); here
+ * sector_data_count must have S+1 entries from 0, non-decreasing, ending at
+ * n_samples; data_indices must be a permutation of [0, n_samples); the
+ * centers must follow the standard layout (index * sw + sw / 2,
+ * :207-270); and every sorted sample must lie in the sector whose range
+ * holds it (CUDA/inc/precomp_utils.hpp:34-48). img_dims[2] == 0 selects 2-D. */
+int gnufft_check_precomputed(const uint32_t img_dims[3], uint32_t sector_width, float osf,
+                             const float* traj_sorted, const uint32_t* data_indices,
+                             const uint32_t* sector_data_count, const uint32_t* sector_centers,
+                             uint32_t n_samples);
 
 /* Coil sensitivities [coil][img], kept on the device (setSens,
  * CUDA/inc/gpuNUFFT_operator.hpp:113-116). NULL / 0 coils clears them. */
@@ -149,7 +166,12 @@ int gnufft_plan_set_sens(gnufft_plan plan, const void* sens, uint32_t n_coils, i
 /* Copy precompute tables to host buffers (any pointer may be NULL):
  * getDataIndices, getSectorDataCount, getSectorCenters, getKSpaceTraj,
  * getDens, deapodization function, getKernel
- * (CUDA/inc/gpuNUFFT_operator.hpp:135-201). */
+ * (CUDA/inc/gpuNUFFT_operator.hpp:135-201). For a plan whose trajectory was
+ * set (not loaded), data_indices, traj_sorted and dens_sorted come in the
+ * reference's own order: std::sort of (index, sector) pairs compared by
+ * sector (CUDA/src/gpuNUFFT_operator_factory.cpp:56-72), built on the host
+ * at this call; the kernels keep the device sort's stable order. A loaded
+ * plan returns the tables it was given. */
 int gnufft_plan_get_tables(gnufft_plan plan, uint32_t* data_indices,
                            uint32_t* sector_data_count, uint32_t* sector_centers,
                            float* traj_sorted, float* dens_sorted, float* deapo, float* lut);
@@ -215,7 +237,11 @@ int gnufft_interp_sorted(gnufft_plan plan, const void* grid, void* data_sorted,
  * coils of those plans.
  * n_coils[i]: coils of plan i (its sensitivity coil count when it has
  * sensitivities). The batch keeps the plans' device tables by reference:
- * plans must outlive it and keep their trajectories. */
+ * plans must outlive it. A plan whose trajectory or sensitivities are set
+ * again after batch_create makes batch_adjoint / batch_forward return
+ * GNUFFT_ERR_STATE (create the batch again). A batch's calls may be issued
+ * on different streams: each call's device work is ordered after the
+ * previous call's (they share the batch's descriptors and work buffers). */
 typedef struct gnufft_batch_s* gnufft_batch;
 int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t n_plans,
                         const uint32_t* n_coils, void* stream);
@@ -236,7 +262,13 @@ int gnufft_batch_forward(gnufft_batch batch, const void* const* img, void* const
  * warm-up that allocates the plan's work buffers) and replayed.
  * The plan must have sensitivities of n_coils coils. solve: kdata [n_coils][M]
  * (device, caller order) -> x_out [img] (device); asynchronous on `stream`.
- * One solver per stream at a time (it owns its vectors). */
+ * One solver per stream at a time (it owns its vectors).
+ * The captured graph holds the plan's device pointers. When the plan frees
+ * or replaces any of them after the capture (set_trajectory,
+ * load_precomputed, set_sens, set_fft, or work buffers grown by a call with
+ * more coils), the next graph solve captures again; a plan whose sample
+ * count or sensitivity coil count differs from cg_create's returns
+ * GNUFFT_ERR_STATE. */
 typedef struct gnufft_cg_s* gnufft_cg;
 int gnufft_cg_create(gnufft_cg* cg, gnufft_plan plan, uint32_t n_coils, float alpha);
 int gnufft_cg_destroy(gnufft_cg cg);

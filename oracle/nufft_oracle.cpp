@@ -14,6 +14,8 @@
 // Pinning status of each function (how a plausible mistake is caught) is
 // listed in oracle/README.md and checked by tests/test_oracle_*.py.
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -370,15 +372,16 @@ static DType axis_weight(const o_info* gi, int lookup, const float* lut, DType d
 // (z, y, x loops), then the sub-grid is added into the grid with wrap.
 // For the linear lookup mode, kernel_count must be the 1-D table length.
 // ---------------------------------------------------------------------------
-void o_spread(const o_info* gi, int lookup, const float* lut, const float* crds,
-              const float* data, unsigned M, unsigned C, const unsigned* sdc,
-              const unsigned* centers, float* grid) {
+// One sector's samples accumulated in the padded sub-grid `sub` (zeroed
+// here) and flushed into `grid` with periodic wrap.
+static void spread_sector(const o_info* gi, int lookup, const float* lut, const float* crds,
+                          const float* data, unsigned M, unsigned C, const unsigned* sdc,
+                          const unsigned* centers, unsigned sec, float* sub, float* grid) {
   const int P = gi->sector_pad_width;
   const int off = gi->sector_offset;
   const int d3 = gi->dim == 3;
-  std::vector<float> sub((size_t)gi->sector_dim * 2 * C);
-  for (unsigned sec = 0; sec < gi->sector_count; sec++) {
-    std::fill(sub.begin(), sub.end(), 0.0f);
+  std::fill(sub, sub + (size_t)gi->sector_dim * 2 * C, 0.0f);
+  {
     IndType cx = centers[(d3 ? 3 : 2) * sec];
     IndType cy = centers[(d3 ? 3 : 2) * sec + 1];
     IndType cz = d3 ? centers[3 * sec + 2] : 0;
@@ -447,6 +450,42 @@ void o_spread(const o_info* gi, int lookup, const float* lut, const float* crds,
   }
 }
 
+void o_spread(const o_info* gi, int lookup, const float* lut, const float* crds,
+              const float* data, unsigned M, unsigned C, const unsigned* sdc,
+              const unsigned* centers, float* grid) {
+  std::vector<float> sub((size_t)gi->sector_dim * 2 * C);
+  for (unsigned sec = 0; sec < gi->sector_count; sec++)
+    spread_sector(gi, lookup, lut, crds, data, M, C, sdc, centers, sec, sub.data(), grid);
+}
+
+// The same spread on `threads` OpenMP threads (CPU baseline only): sectors
+// are shared out dynamically, each thread flushes into a private grid of
+// all coils, and the private grids are summed at the end (in thread order).
+void o_spread_mt(const o_info* gi, int lookup, const float* lut, const float* crds,
+                 const float* data, unsigned M, unsigned C, const unsigned* sdc,
+                 const unsigned* centers, float* grid, int threads) {
+  if (threads < 1) threads = 1;
+  const size_t n = (size_t)gi->grid_count * 2 * C;
+  std::vector<std::vector<float>> priv((size_t)threads);
+#pragma omp parallel num_threads(threads)
+  {
+    const int t = omp_get_thread_num();
+    std::vector<float>& g = priv[(size_t)t];
+    g.assign(n, 0.0f);
+    std::vector<float> sub((size_t)gi->sector_dim * 2 * C);
+#pragma omp for schedule(dynamic, 4)
+    for (long sec = 0; sec < (long)gi->sector_count; sec++)
+      spread_sector(gi, lookup, lut, crds, data, M, C, sdc, centers, (unsigned)sec, sub.data(),
+                    g.data());
+#pragma omp for schedule(static)
+    for (long i = 0; i < (long)n; i++) {
+      float acc = grid[i];
+      for (int k = 0; k < threads; k++) acc += priv[(size_t)k][(size_t)i];
+      grid[i] = acc;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Forward interpolation (E7): sample = sum over taps of weight*grid value,
 // taps in (z, y, x) order, grid read with periodic wrap.
@@ -454,12 +493,13 @@ void o_spread(const o_info* gi, int lookup, const float* lut, const float* crds,
 // :1234-1328 (2-D); texture mode texture_gpuNUFFT_kernels.cu:467-570, 636-739.
 // out complex [C][M] in sector order (overwritten).
 // ---------------------------------------------------------------------------
-void o_interp(const o_info* gi, int lookup, const float* lut, const float* crds,
-              const float* grid, unsigned M, unsigned C, const unsigned* sdc,
-              const unsigned* centers, float* out) {
+// One sector's samples interpolated from `grid` into `out`.
+static void interp_sector(const o_info* gi, int lookup, const float* lut, const float* crds,
+                          const float* grid, unsigned M, unsigned C, const unsigned* sdc,
+                          const unsigned* centers, unsigned sec, float* out) {
   const int off = gi->sector_offset;
   const int d3 = gi->dim == 3;
-  for (unsigned sec = 0; sec < gi->sector_count; sec++) {
+  {
     IndType cx = centers[(d3 ? 3 : 2) * sec];
     IndType cy = centers[(d3 ? 3 : 2) * sec + 1];
     IndType cz = d3 ? centers[3 * sec + 2] : 0;
@@ -515,6 +555,24 @@ void o_interp(const o_info* gi, int lookup, const float* lut, const float* crds,
       }
     }
   }
+}
+
+void o_interp(const o_info* gi, int lookup, const float* lut, const float* crds,
+              const float* grid, unsigned M, unsigned C, const unsigned* sdc,
+              const unsigned* centers, float* out) {
+  for (unsigned sec = 0; sec < gi->sector_count; sec++)
+    interp_sector(gi, lookup, lut, crds, grid, M, C, sdc, centers, sec, out);
+}
+
+// The same interpolation on `threads` OpenMP threads (CPU baseline only):
+// every sample is written by one sector, so sectors run independently.
+void o_interp_mt(const o_info* gi, int lookup, const float* lut, const float* crds,
+                 const float* grid, unsigned M, unsigned C, const unsigned* sdc,
+                 const unsigned* centers, float* out, int threads) {
+  if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4)
+  for (long sec = 0; sec < (long)gi->sector_count; sec++)
+    interp_sector(gi, lookup, lut, crds, grid, M, C, sdc, centers, (unsigned)sec, out);
 }
 
 // ---------------------------------------------------------------------------

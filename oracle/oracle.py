@@ -102,6 +102,8 @@ def lib():
                                ctypes.c_uint, u32p, u32p, f32p]
         L.o_interp.argtypes = [P(OInfo), ctypes.c_int, f32p, f32p, f32p, ctypes.c_uint,
                                ctypes.c_uint, u32p, u32p, f32p]
+        L.o_spread_mt.argtypes = L.o_spread.argtypes + [ctypes.c_int]
+        L.o_interp_mt.argtypes = L.o_interp.argtypes + [ctypes.c_int]
         L.o_cpu_gridding.argtypes = [f32p, f32p, f32p, f32p, i32p, ctypes.c_int, i32p,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -246,31 +248,40 @@ def precompute(traj, img_dims, osf, kw, sw, dens=None, lookup=LOOKUP_NEAREST, or
 # Gridding
 # ---------------------------------------------------------------------------
 
-def spread(gi, lookup, lut, crds_sorted, data_sorted, sdc, centers):
-    """data_sorted: complex64 (C, M) in sector order -> grid complex64 (C, *grid)."""
+def spread(gi, lookup, lut, crds_sorted, data_sorted, sdc, centers, threads=1):
+    """data_sorted: complex64 (C, M) in sector order -> grid complex64 (C, *grid).
+    threads > 1: OpenMP over sectors with private grids (CPU baseline)."""
     crds = _f32(crds_sorted)
     data = np.ascontiguousarray(data_sorted, dtype=np.complex64)
     if data.ndim == 1:
         data = data[None]
     C, M = data.shape
     grid = np.zeros((C,) + grid_shape(gi), np.complex64)
-    lib().o_spread(ctypes.byref(gi), lookup, _p(_f32(lut), ctypes.c_float),
-                   _p(crds, ctypes.c_float), _p(data.view(np.float32), ctypes.c_float), M, C,
-                   _p(_u32(sdc), ctypes.c_uint), _p(_u32(centers), ctypes.c_uint),
-                   _p(grid.view(np.float32), ctypes.c_float))
+    args = (ctypes.byref(gi), lookup, _p(_f32(lut), ctypes.c_float),
+            _p(crds, ctypes.c_float), _p(data.view(np.float32), ctypes.c_float), M, C,
+            _p(_u32(sdc), ctypes.c_uint), _p(_u32(centers), ctypes.c_uint),
+            _p(grid.view(np.float32), ctypes.c_float))
+    if threads > 1:
+        lib().o_spread_mt(*args, int(threads))
+    else:
+        lib().o_spread(*args)
     return grid
 
 
-def interp(gi, lookup, lut, crds_sorted, grid, sdc, centers):
+def interp(gi, lookup, lut, crds_sorted, grid, sdc, centers, threads=1):
     crds = _f32(crds_sorted)
     grid = np.ascontiguousarray(grid, dtype=np.complex64)
     C = grid.shape[0]
     M = crds.shape[1]
     out = np.zeros((C, M), np.complex64)
-    lib().o_interp(ctypes.byref(gi), lookup, _p(_f32(lut), ctypes.c_float),
-                   _p(crds, ctypes.c_float), _p(grid.view(np.float32), ctypes.c_float), M, C,
-                   _p(_u32(sdc), ctypes.c_uint), _p(_u32(centers), ctypes.c_uint),
-                   _p(out.view(np.float32), ctypes.c_float))
+    args = (ctypes.byref(gi), lookup, _p(_f32(lut), ctypes.c_float),
+            _p(crds, ctypes.c_float), _p(grid.view(np.float32), ctypes.c_float), M, C,
+            _p(_u32(sdc), ctypes.c_uint), _p(_u32(centers), ctypes.c_uint),
+            _p(out.view(np.float32), ctypes.c_float))
+    if threads > 1:
+        lib().o_interp_mt(*args, int(threads))
+    else:
+        lib().o_interp(*args)
     return out
 
 
@@ -321,12 +332,20 @@ def forward_shift(gi, g):
     return np.fft.fftshift(g, axes=_spatial_axes(gi))
 
 
-def ifft_unnormalised(gi, g):
+def ifft_unnormalised(gi, g, threads=1):
+    if threads > 1:
+        import scipy.fft
+        return scipy.fft.ifftn(g.astype(np.complex128), axes=_spatial_axes(gi), norm="forward",
+                               workers=threads).astype(np.complex64)
     return np.fft.ifftn(g.astype(np.complex128), axes=_spatial_axes(gi), norm="forward").astype(
         np.complex64)
 
 
-def fft_unnormalised(gi, g):
+def fft_unnormalised(gi, g, threads=1):
+    if threads > 1:
+        import scipy.fft
+        return scipy.fft.fftn(g.astype(np.complex128), axes=_spatial_axes(gi),
+                              workers=threads).astype(np.complex64)
     return np.fft.fftn(g.astype(np.complex128), axes=_spatial_axes(gi)).astype(np.complex64)
 
 
@@ -362,8 +381,9 @@ class Operator:
     """
 
     def __init__(self, traj, img_dims, kernel_width, sector_width, osf, dens=None, sens=None,
-                 lookup=LOOKUP_NEAREST, order="std", deapo=True, deapo_norm="image"):
+                 lookup=LOOKUP_NEAREST, order="std", deapo=True, deapo_norm="image", threads=1):
         self.lookup = lookup
+        self.threads = int(threads)  # > 1: OpenMP gridding and threaded FFTs (CPU baseline)
         self.kw, self.sw, self.osf = kernel_width, sector_width, np.float32(osf)
         self.pre = precompute(traj, img_dims, float(osf), kernel_width, sector_width, dens,
                               lookup, order)
@@ -401,11 +421,11 @@ class Operator:
         if self.apply_dens():
             sorted_k = _cmul_real(sorted_k, self._sqrt_dens_sorted()[None])
         grid = spread(gi, self.lookup, self.lut, pre["traj_sorted"], sorted_k,
-                      pre["sector_data_count"], pre["sector_centers"])
+                      pre["sector_data_count"], pre["sector_centers"], self.threads)
         if stop in (CONVOLUTION, DENSITY_ESTIMATION):
             return grid
         g = inverse_shift(gi, grid)
-        g = ifft_unnormalised(gi, g)
+        g = ifft_unnormalised(gi, g, self.threads)
         g = inverse_shift(gi, g)
         img = crop(gi, g)
         img = _cmul_real(img, fft_scale(gi))
@@ -439,10 +459,10 @@ class Operator:
         else:
             g = pad(gi, x)
             g = inverse_shift(gi, g)
-            g = fft_unnormalised(gi, g)
+            g = fft_unnormalised(gi, g, self.threads)
             g = forward_shift(gi, g)
             samples = interp(gi, self.lookup, self.lut, pre["traj_sorted"], g,
-                             pre["sector_data_count"], pre["sector_centers"])
+                             pre["sector_data_count"], pre["sector_centers"], self.threads)
             samples = _cmul_real(samples, fft_scale(gi))
             if self.apply_dens():
                 samples = _cmul_real(samples, self._sqrt_dens_sorted()[None])

@@ -123,6 +123,61 @@ def test_batch_equals_single_plans(oracle, env, make, fft):
     assert oracle.rel_l2(f_out[j].cpu().numpy(), op.forward(img[j].cpu().numpy())) < TOL
 
 
+def test_uneven_batch_full_size_autotuned(oracle, env):
+    """The benchmark's batch at its real shape: all 64 operators of the
+    uneven 2-D workload (N 96-384, 4-16 coils, spirals of 8-48 x
+    1024-4096 samples, sensitivities and density), sector widths chosen by
+    the autotuner from {4, 8, 16} at full-GPU occupancy, with one operator
+    forced to 4 and one to 8 so the batch mixes all three widths. Every
+    operator's batched adjoint and forward equal its single-plan results
+    (rel-L2 <= 1e-5); three operators are also checked against the oracle:
+    an N = 384 operator with 16 coils, and the sw 4 and sw 8 ones."""
+    torch, capi = env
+    from gpunufft_amd import autotune
+    ops = W.uneven_batch_2d()
+    big = next(j for j, o in enumerate(ops) if o["img_dims"][0] == 384 and o["coils"] == 16)
+    forced = {1 if big != 1 else 2: 4, 3 if big != 3 else 4: 8}
+    plans, sws, kd, img = [], [], [], []
+    for j, o in enumerate(ops):
+        t_d = _dev(torch, o["traj"])
+        d_d = _dev(torch, o["dens"])
+        sw = forced.get(j) or autotune.best_sector_width(o["img_dims"], o["kw"], o["osf"], t_d,
+                                                         d_d, o["coils"], (4, 8, 16))
+        sws.append(sw)
+        p = capi.Plan(o["img_dims"], o["kw"], sw, o["osf"])
+        p.set_trajectory(t_d, d_d)
+        p.set_sens(_dev(torch, o["sens"]), o["coils"])
+        plans.append(p)
+        rng = np.random.default_rng(o["seed"])
+        N = o["img_dims"][0]
+        kd.append(_dev(torch, W.complex_normal(rng, (o["coils"], o["traj"].shape[1]))))
+        img.append(_dev(torch, W.complex_normal(rng, (1, N, N))))
+    print("widths", sws, flush=True)
+    assert set(sws) == {4, 8, 16} or set(sws) >= {4, 8}
+    b = capi.Batch(plans, [o["coils"] for o in ops])
+    a_out = [torch.empty((1,) + o["img_dims"], dtype=torch.complex64, device="cuda") for o in ops]
+    f_out = [torch.empty(k.shape, dtype=torch.complex64, device="cuda") for k in kd]
+    b.adjoint(kd, a_out)
+    b.forward(img, f_out)
+    torch.cuda.synchronize()
+    for j, (p, o) in enumerate(zip(plans, ops)):
+        ref_a = torch.empty_like(a_out[j])
+        p.adjoint(kd[j], ref_a, o["coils"])
+        ref_f = torch.empty_like(f_out[j])
+        p.forward(img[j], ref_f, o["coils"])
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(a_out[j].cpu().numpy(), ref_a.cpu().numpy()) < TOL, ("adjoint", j, sws[j])
+        assert oracle.rel_l2(f_out[j].cpu().numpy(), ref_f.cpu().numpy()) < TOL, ("forward", j, sws[j])
+    for j in [big] + sorted(forced):
+        o = ops[j]
+        print("oracle check", j, o["img_dims"], o["coils"], sws[j], flush=True)
+        op = oracle.Operator(o["traj"], o["img_dims"], o["kw"], sws[j], o["osf"], dens=o["dens"],
+                             sens=o["sens"], order="stable")
+        assert oracle.rel_l2(a_out[j].cpu().numpy(), op.adjoint(kd[j].cpu().numpy())) < TOL, j
+        assert oracle.rel_l2(f_out[j].cpu().numpy(), op.forward(img[j].cpu().numpy())) < TOL, j
+    b.close()
+
+
 def test_batch_rejects_mixed_dimensions(env):
     torch, capi = env
     p2 = capi.Plan((16, 16), 3, 8, 2.0)

@@ -93,6 +93,74 @@ def test_graph_replay_equals_eager(oracle, env):
         torch.cuda.synchronize()
 
 
+def test_graph_recaptured_after_plan_changes(oracle, env):
+    """The captured iteration holds the plan's device pointers: after
+    set_sens (new sensitivities) and after set_trajectory (new tables of the
+    same length), the next graph solve captures again and equals an eager
+    solve of the changed plan (rel-L2 <= 1e-6)."""
+    torch, capi = env
+    dims = (40, 40)
+    traj, dens, sens, kd = _problem(dims, C=3, seed=5)
+    pl, cg, xg, kdev = _gpu_cg(torch, capi, dims, traj, dens, sens, kd, 6, 1e-4, True)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    cg.solve(kdev, xg, 6, use_graph=True)
+    torch.cuda.synchronize()
+    first = xg.cpu().numpy()
+    # new sensitivities
+    sens2 = W.gaussian_sens(dims, 3, seed=9)
+    pl.set_sens(dev(sens2), 3)
+    xg.zero_()
+    cg.solve(kdev, xg, 6, use_graph=True)
+    torch.cuda.synchronize()
+    graph2 = xg.cpu().numpy()
+    cg.solve(kdev, xg, 6, use_graph=False)
+    torch.cuda.synchronize()
+    eager2 = xg.cpu().numpy()
+    print("sens change: graph vs eager", oracle.rel_l2(graph2, eager2), flush=True)
+    assert oracle.rel_l2(graph2, eager2) < 1e-6
+    assert oracle.rel_l2(graph2, first) > 1e-3  # the new maps took effect
+    # new trajectory of the same length (rotated spokes)
+    th = 0.37
+    rot = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]], np.float32)
+    traj2 = np.clip(rot @ traj, -0.5, 0.4999).astype(np.float32)
+    pl.set_trajectory(dev(traj2), dev(dens))
+    xg.zero_()
+    cg.solve(kdev, xg, 6, use_graph=True)
+    torch.cuda.synchronize()
+    graph3 = xg.cpu().numpy()
+    cg.solve(kdev, xg, 6, use_graph=False)
+    torch.cuda.synchronize()
+    eager3 = xg.cpu().numpy()
+    print("trajectory change: graph vs eager", oracle.rel_l2(graph3, eager3), flush=True)
+    assert oracle.rel_l2(graph3, eager3) < 1e-6
+    # a different sample count cannot reuse the solver's vectors
+    pl.set_trajectory(dev(traj[:, :-8].copy()), dev(dens[:-8].copy()))
+    with pytest.raises(capi.GnufftError) as ei:
+        cg.solve(kdev, xg, 2, use_graph=True)
+    assert ei.value.code == capi.ERR_STATE
+
+
+def test_batch_rejects_changed_plan(env):
+    torch, capi = env
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    traj = W.radial_2d(16, 32)
+    plans = []
+    for n in (24, 32):
+        p = capi.Plan((n, n), 3, 8, 2.0)
+        p.set_trajectory(dev(traj))
+        plans.append(p)
+    b = capi.Batch(plans, [2, 2])
+    kd = [dev(W.complex_normal(np.random.default_rng(0), (2, traj.shape[1]))) for _ in plans]
+    out = [torch.empty((2, n, n), dtype=torch.complex64, device="cuda") for n in (24, 32)]
+    b.adjoint(kd, out)
+    torch.cuda.synchronize()
+    plans[1].set_trajectory(dev(traj))
+    with pytest.raises(capi.GnufftError) as ei:
+        b.adjoint(kd, out)
+    assert ei.value.code == capi.ERR_STATE
+    b.close()
+
+
 def test_serving_mix_equals_alone(oracle, env):
     torch, capi = env
     from gpunufft_amd import serve

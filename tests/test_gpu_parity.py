@@ -49,7 +49,14 @@ CASES = [
      (20, 18, 11), 1.5, 3, 8),
     ("rand3d_kw4_sw6", lambda: np.random.default_rng(8).uniform(-0.5, 0.5, (3, 1500)).astype(np.float32),
      (16, 16, 16), 2.0, 4, 6),
+    # the sector widths the autotuner chooses from (4, 8, 16): P = 6 and 18
+    ("spiral2d_sw16", lambda: W.spiral_2d(12, 1500, n_turns=9.6), (96, 96), 2.0, 3, 16),
+    ("spiral2d_sw4", lambda: W.spiral_2d(8, 1200, n_turns=6.4), (64, 64), 2.0, 3, 4),
+    ("koosh3d_sw16", lambda: W.koosh_3d(400, 48), (32, 32, 32), 1.5, 3, 16),
+    ("rand3d_sw4", lambda: np.random.default_rng(9).uniform(-0.5, 0.5, (3, 3000)).astype(np.float32),
+     (20, 20, 20), 1.5, 3, 4),
 ]
+SW_CASES = CASES[7:]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
@@ -66,14 +73,14 @@ def test_precompute_tables_bit_exact(oracle, torch_dev, capi, case):
     assert np.array_equal(pl.assigned_sectors(), ref["sectors"])
     assert np.array_equal(t["sector_data_count"], ref["sector_data_count"])
     assert np.array_equal(t["sector_centers"], ref["sector_centers"])
-    # canonical (stable) order; same per-sector index sets as the reference std::sort
-    assert np.array_equal(t["data_indices"], stable["data_indices"])
+    # the getters report the reference's std::sort permutation, bit-exact
+    assert np.array_equal(t["data_indices"], ref["data_indices"])
+    assert np.array_equal(t["traj_sorted"], ref["traj_sorted"])
+    assert np.array_equal(t["dens_sorted"], ref["dens_sorted"])
+    # sectors of more than 16 samples are where std::sort leaves the stable order
     sdc = ref["sector_data_count"]
-    for s in np.nonzero(np.diff(sdc))[0][:200]:
-        a, b = sdc[s], sdc[s + 1]
-        assert set(t["data_indices"][a:b]) == set(ref["data_indices"][a:b])
-    assert np.array_equal(t["traj_sorted"], stable["traj_sorted"])
-    assert np.array_equal(t["dens_sorted"], stable["dens_sorted"])
+    if np.max(np.diff(sdc)) > 16:
+        assert not np.array_equal(ref["data_indices"], stable["data_indices"])
 
 
 @pytest.mark.parametrize("lookup", [0, 1])
@@ -113,7 +120,7 @@ def _gridding_case(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, s
 
 @pytest.mark.parametrize("lookup", [0, 1])
 @pytest.mark.parametrize("C", [1, 3, 8, 11])
-@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3], CASES[4], CASES[5], CASES[6]],
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3], CASES[4], CASES[5], CASES[6]] + SW_CASES,
                          ids=lambda c: c[0])
 def test_spread_interp_kernels(oracle, torch_dev, capi, case, C, lookup):
     name, tf, img, osf, kw, sw = case
@@ -141,7 +148,7 @@ def _op_pair(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, dens, s
 
 @pytest.mark.parametrize("lookup", [0, 1])
 @pytest.mark.parametrize("use_dens,use_sens", [(False, False), (True, False), (True, True)])
-@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2], CASES[4], CASES[5]],
+@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2], CASES[4], CASES[5]] + SW_CASES,
                          ids=lambda c: c[0])
 def test_operator_all_stages(oracle, torch_dev, capi, case, use_dens, use_sens, lookup):
     name, tf, img, osf, kw, sw = case

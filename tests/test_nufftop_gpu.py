@@ -120,8 +120,8 @@ def test_nufftop_tables(oracle, NUFFTOp, balance):
     dens = W.ramp_density(traj)
     op = NUFFTOp(traj, np.array(img, np.int32), 1, None, dens, 3, 8, 2, balance)
     t = op.get_tables()
-    pre = oracle.precompute(traj, img, 2.0, 3, 8, dens, lookup=oracle.LOOKUP_LINEAR,
-                            order="stable")
+    # the reference's std::sort permutation, bit-exact
+    pre = oracle.precompute(traj, img, 2.0, 3, 8, dens, lookup=oracle.LOOKUP_LINEAR, order="std")
     assert t["operator_type"] == (3 if balance else 1)  # BALANCED_TEXTURE / TEXTURE
     assert np.array_equal(t["data_indices"], pre["data_indices"])
     assert np.array_equal(t["sector_data_count"], pre["sector_data_count"])
