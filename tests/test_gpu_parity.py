@@ -321,6 +321,37 @@ def test_reference_fixtures_on_gpu(oracle, fixtures, torch_dev, capi):
     assert abs(host(out)[0, 0, 0].real - f["expect"]) < f["eps"]
 
 
+@pytest.mark.parametrize("gridding", [0, 1], ids=["sectors", "tiles"])
+@pytest.mark.parametrize("key", ["gpu_conv", "gpu_fft_stop"])
+def test_reference_gpu_known_answers(oracle, fixtures, torch_dev, capi, key, gridding):
+    """The reference's GPU convolution / FFT-stage known answers
+    (gpuNUFFT_conv_tests.cpp, gpuNUFFT_2D_conv_tests.cpp,
+    gpuNUFFT_fft_tests.cpp, transcribed in tests/golden) on the HIP path:
+    every value within the tolerance the reference asserts, and within 1e-5
+    (relative to the largest grid value) of the oracle at the same point."""
+    from test_oracle_fixtures import reference_gpu_fixture_run
+    stop = oracle.CONVOLUTION if key == "gpu_conv" else oracle.FFT
+    for f in fixtures[key]:
+        print("fixture", f["src"], flush=True)
+        traj = np.array(f["traj"], np.float32)
+        data = np.array([complex(*v) for v in f["data"]], np.complex64)[None]
+        pl = capi.Plan(tuple(f["img"]), f["kw"], f["sw"], f["osf"], 0)
+        pl.set_trajectory(dev(torch_dev, traj))
+        pl.set_gridding(gridding)
+        gi = oracle.make_info(tuple(f["img"]), f["osf"], f["kw"], f["sw"])
+        shape = (1,) + (oracle.grid_shape(gi) if stop == oracle.CONVOLUTION else oracle.img_shape(gi))
+        out = torch_dev.empty(shape, dtype=torch_dev.complex64, device="cuda")
+        pl.adjoint(dev(torch_dev, data), out, 1, stop)
+        got = host(out)[0]
+        scale = float(np.max(np.abs(got)))
+        for ck, ref in reference_gpu_fixture_run(oracle, f, stop):
+            x, y, z, comp, exp, eps = ck
+            v = got[z, y, x] if got.ndim == 3 else got[y, x]
+            v = float(v.real if comp == 0 else v.imag)
+            assert abs(v - exp) <= eps, (f["src"], ck, v)
+            assert abs(v - ref) <= 1e-5 * scale, (f["src"], ck, v, ref)
+
+
 def _adjointness(torch_dev, pl, img_shape, C, M, use_sens, seed):
     torch = torch_dev
     g = torch.Generator(device="cuda").manual_seed(seed)
