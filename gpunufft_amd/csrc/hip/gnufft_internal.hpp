@@ -132,6 +132,18 @@ __device__ __host__ inline bool axis_weight_kp(const Geom& g, const float* lut, 
   return true;
 }
 
+// Sector index along one axis (CUDA/inc/precomp_utils.hpp:34-48): float
+// product, double sum, round half away from zero in double, double
+// quotient, floor, clamped to ceil((float)G/sw)-1 evaluated in float.
+__device__ inline int sector_of(float coord, int G, int sw) {
+  double v = (double)(coord * (float)G) + 0.5 * (double)(float)G;
+  int sector = (int)floor(round(v) / (double)(float)sw);
+  int smax = (int)ceilf((float)G / (float)sw);
+  if (sector >= smax) sector = smax - 1;
+  if (sector < 0) sector = 0;
+  return sector;
+}
+
 #pragma clang fp contract(on)
 
 // Periodic wrap into [0, G). Callers pass coordinates within a few grid

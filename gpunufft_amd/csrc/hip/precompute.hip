@@ -14,22 +14,6 @@
 
 namespace gnufft {
 
-#pragma clang fp contract(off)
-
-// Sector index along one axis (precomp_utils.hpp:34-48): float product,
-// double sum, round half away from zero in double, double quotient, floor,
-// clamped to ceil((float)G/sw)-1 evaluated in float.
-__device__ inline int sector_of(float coord, int G, int sw) {
-  double v = (double)(coord * (float)G) + 0.5 * (double)(float)G;
-  int sector = (int)floor(round(v) / (double)(float)sw);
-  int smax = (int)ceilf((float)G / (float)sw);
-  if (sector >= smax) sector = smax - 1;
-  if (sector < 0) sector = 0;
-  return sector;
-}
-
-#pragma clang fp contract(on)
-
 __global__ void __launch_bounds__(256) assign_sectors_kernel(Geom g, const float* __restrict__ traj,
                                                              uint32_t M, uint32_t* __restrict__ keys,
                                                              uint32_t* __restrict__ vals) {

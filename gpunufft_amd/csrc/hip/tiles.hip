@@ -17,14 +17,15 @@
 // land inside the tile, so every (sample, tap) pair is accumulated by
 // exactly one workgroup.
 //
-// Tap windows and weights are computed from the sample's global grid
-// position A = k * G + G / 2: the taps are ceil(A - kw/2) .. floor(A + kw/2)
-// and the weight of global tap g uses the k-space position g / G - 1/2,
-// the same float operations as the reference's sector-relative form
-// (map_grid_to_kspace, CUDA/inc/gpuNUFFT_utils.hpp:202-209, with
-// i + center - offset == g exactly), so every weight is bit-identical. The
-// window can differ from the reference's sector-relative window only by a
-// tap at distance kw/2, whose weight is zero in both lookup modes.
+// Every visit applies exactly the reference's taps: the window is the
+// reference's, relative to the sample's own sector (E3 assignment, padded
+// sector window with its clamps, CUDA/inc/precomp_utils.hpp:34-48 and
+// CUDA/inc/gpuNUFFT_utils.hpp:89-104, 194-199), carried to global grid
+// indices; it matters at the top grid edge, where samples clamped into the
+// last sector lose their taps past the padded window. The weight of global
+// tap g uses the k-space position g / G - 1/2, the same float operations as
+// the reference's map_grid_to_kspace (gpuNUFFT_utils.hpp:202-209) with
+// i + center - offset == g exactly, so every weight is bit-identical.
 //
 // Tiles with more visits than one workgroup should take are split into
 // parts; each part writes its partial tile to scratch and a reduction sums
@@ -72,12 +73,19 @@ __device__ __host__ inline float grid_pos(float pos, int G) {
 // i + center - offset == g.
 __device__ __host__ inline float kspace_of(int g, int G) { return (float)g / (float)G - 0.5f; }
 
-// Taps ceil(A - r) .. floor(A + r), at most kw + 1 of them (the precompute
-// and the kernel use this one definition, so they agree on every window).
-__device__ inline void tap_window(float A, float r, int kw, int* lo, int* hi) {
-  *lo = (int)ceilf(A - r);
-  *hi = (int)floorf(A + r);
-  if (*hi > *lo + kw) *hi = *lo + kw;
+// The reference's taps of one axis as global (unwrapped) grid indices
+// lo .. hi: the sample's sector (E3), its padded window and the clamped tap
+// range inside it. The precompute and the kernel use this one definition,
+// so they agree on every window.
+__device__ inline void ref_taps(const Geom& g, int a, float pos, int* lo, int* hi) {
+  const int G = g.G[a];
+  const int c = sector_of(pos, G, g.sw) * g.sw + g.sw / 2;
+  const float u = map_kspace_to_grid(pos, G, c, g.off);
+  int l, h;
+  tap_range(u, g.Pmax, g.kradius, &l, &h);
+  if (h > l + g.kw) h = l + g.kw;
+  *lo = l + c - g.off;
+  *hi = h + c - g.off;
 }
 
 __device__ inline int home_coord(float A) { return (int)floorf(A + 0.5f); }
@@ -95,7 +103,7 @@ __device__ inline int axis_tiles(const Geom& g, const TileGrid& t, int a, float 
   const int G = g.G[a];
   const float A = grid_pos(pos, G);
   int lo, hi;
-  tap_window(A, g.kradius, g.kw, &lo, &hi);
+  ref_taps(g, a, pos, &lo, &hi);
   const int home = tile_of_coord(t.bound[a], t.n[a], wrap(home_coord(A), G));
   int n = 0;
   *home_pos = 0;
@@ -333,9 +341,8 @@ __device__ inline void make_tile_record(const Geom& g, const float* lut, const i
 #pragma unroll
   for (int a = 0; a < DIM; a++) {
     const int G = g.G[a];
-    const float A = grid_pos(p[a], G);
     int lo, hi;
-    tap_window(A, g.kradius, g.kw, &lo, &hi);
+    ref_taps(g, a, p[a], &lo, &hi);
     uint32_t idx[IW];
 #pragma unroll
     for (int k = 0; k < IW; k++) idx[k] = 0;

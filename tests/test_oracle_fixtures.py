@@ -290,3 +290,112 @@ def test_ndft_error(oracle, dim, img, osf, kw):
     ref = oracle.ndft_adjoint(op.gi, traj, y) * math.sqrt(op.gi.grid_count / op.gi.img_count)
     err = oracle.rel_l2(x, ref)
     assert err < (0.05 if kw == 3 else 0.01), err
+
+
+def _tex2d(T2, u, v, frac_rounding):
+    """Brute-force model of a clamp-addressed, normalized-coordinate,
+    linear-filtered 2-D texture fetch (CUDA Programming Guide, texture
+    fetching / linear filtering; as the reference binds its 2-D kernel table,
+    CUDA/src/gpu/std_gpuNUFFT_kernels.cu:49-60, read by
+    CUDA/inc/cuda_utils.cuh:27-72): texel coordinate c * K - 0.5, the four
+    neighbouring texels clamped to [0, K-1], fractional weights in 9-bit fixed
+    point with 8 fractional bits. Evaluated in float64 over the 2-D table
+    itself (not the separable 1-D form the restatement uses)."""
+    K = T2.shape[0]
+
+    def axis(c):
+        xb = np.float64(c) * K - 0.5
+        i = math.floor(xb)
+        a = xb - i
+        a = (round(a * 256.0) if frac_rounding == "nearest" else math.floor(a * 256.0)) / 256.0
+        return min(max(i, 0), K - 1), min(max(i + 1, 0), K - 1), a
+
+    i0, i1, a = axis(u)  # u: first table index
+    j0, j1, b = axis(v)
+    T = T2.astype(np.float64)
+    return ((1 - a) * (1 - b) * T[i0, j0] + a * (1 - b) * T[i1, j0] + (1 - a) * b * T[i0, j1] +
+            a * b * T[i1, j1])
+
+
+def _spread_linear_brute(oracle, gi, T2, traj, data, frac_rounding):
+    """Adjoint spread in texture mode by direct loops over samples and
+    global grid taps, periodic wrap, weight tex2D(x^2, y^2) [* tex2D(z^2, 0)]
+    as texture_gpuNUFFT_kernels.cu:63-95 / cuda_utils.cuh:45-55 form it (no
+    radius test in this mode). Taps: the clamped range of the sample's padded
+    sector window (precomp_utils.hpp:34-48, gpuNUFFT_utils.hpp:89-104,
+    194-199), which drops taps of samples clamped into the last sector."""
+    d = gi.dim
+    G = [gi.grid[a] for a in range(d)]
+    Gmax = max(G)
+    r = gi.kernel_width / 2.0
+    rinv = 1.0 / (r / Gmax) ** 2
+    grid = np.zeros([G[a] for a in reversed(range(d))], np.complex128)
+    sw = gi.sector_width
+    P = sw + 2 * (gi.kernel_width // 2)
+    off, pmax = P // 2, P - 1
+    f32 = np.float32
+    for s in range(traj.shape[1]):
+        taps = []
+        for a in range(d):
+            # the sample's sector (half away from zero, clamped to the last
+            # sector) and the clamped tap range of its padded window
+            v = float(f32(traj[a, s]) * f32(G[a])) + 0.5 * G[a]
+            rv = math.floor(v + 0.5) if v >= 0 else -math.floor(-v + 0.5)
+            sec = min(max(math.floor(rv / sw), 0), math.ceil(G[a] / sw) - 1)
+            c = sec * sw + sw // 2
+            u = float(f32(f32(f32(f32(traj[a, s]) * f32(G[a])) + f32(0.5 * G[a])) - f32(c)) + f32(off))
+            mn, mx = math.ceil(u - r), math.floor(u + r)
+            mn, mx = max(mn, 0), min(max(mx, 0), pmax)
+            if mn >= pmax:
+                mn = int(pmax - 2 * r)
+            ax = []
+            for g in range(mn + c - off, mx + c - off + 1):
+                kp = g / G[a] - 0.5
+                dd = ((kp - float(traj[a, s])) * G[a] / Gmax) ** 2 * rinv
+                ax.append((g % G[a], dd))
+            taps.append(ax)
+        if d == 2:
+            for gy, dy in taps[1]:
+                for gx, dx in taps[0]:
+                    grid[gy, gx] += _tex2d(T2, dx, dy, frac_rounding) * complex(data[s])
+        else:
+            for gz, dz in taps[2]:
+                wz = _tex2d(T2, dz, 0.0, frac_rounding)
+                for gy, dy in taps[1]:
+                    for gx, dx in taps[0]:
+                        grid[gz, gy, gx] += _tex2d(T2, dx, dy, frac_rounding) * wz * complex(data[s])
+    return grid
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_linear_lookup_brute_force(oracle, dim):
+    """Texture (linear) lookup mode of the restatement against an
+    independent brute force of the texture unit's fetch on the actual small
+    tables (45^2 entries at osf 2, 60^2 at osf 1.5): clamp addressing at both
+    table ends, the -0.5 texel centre, and 8-bit fractional weights all
+    agree (rel-L2 < 1e-5 between float32 separable lerps and float64
+    bilinear). The one choice no public document fixes is how the hardware
+    rounds the fraction to 8 bits; round-to-nearest (used here) and
+    truncation differ measurably (> 1e-4), so that choice is the only
+    unpinned part of this mode."""
+    rng = np.random.default_rng(21 + dim)
+    img, osf = ((12, 10), 2.0) if dim == 2 else ((8, 6, 6), 1.5)
+    M = 60
+    traj = rng.uniform(-0.5, 0.5, (dim, M)).astype(np.float32)
+    traj[:, 0] = 0.0          # taps at distance 0: u = 0, clamped low end
+    traj[0, 1] = -0.5         # grid edge: wrapped taps
+    traj[0, 2] = 0.49         # top sliver: clamped into the last sector
+    data = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    pre = oracle.precompute(traj, img, osf, 3, 4, lookup=oracle.LOOKUP_LINEAR, order="stable")
+    gi = pre["info"]
+    lut = oracle.load_lut(osf, 3, oracle.LOOKUP_LINEAR)
+    T2 = oracle.load_lut_2d(osf, 3)
+    assert np.array_equal(T2[0], lut) and np.allclose(T2, np.outer(lut, lut), rtol=1e-6, atol=0)
+    got = oracle.spread(gi, oracle.LOOKUP_LINEAR, lut, pre["traj_sorted"],
+                        data[pre["data_indices"]][None], pre["sector_data_count"],
+                        pre["sector_centers"])[0]
+    brute = _spread_linear_brute(oracle, gi, T2, traj, data, "nearest")
+    err = oracle.rel_l2(got, brute.astype(np.complex64))
+    assert err < 1e-5, err
+    trunc = _spread_linear_brute(oracle, gi, T2, traj, data, "truncate")
+    assert oracle.rel_l2(trunc.astype(np.complex64), brute.astype(np.complex64)) > 1e-4
