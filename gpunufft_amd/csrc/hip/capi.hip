@@ -682,10 +682,19 @@ SpreadArgs spread_args(gnufft_plan p, const float2* data, float2* grid, uint32_t
 // visit lists (own samples and neighbours within reach of each tile) and
 // the work items (tiles with more than kTileChunk visits are split into
 // parts whose partial tiles a reduction sums).
-constexpr uint32_t kTileChunk = 4096;
+#ifndef GNUFFT_TILE_CHUNK
+#define GNUFFT_TILE_CHUNK 4096
+#endif
+#ifndef GNUFFT_TILE_3D
+#define GNUFFT_TILE_3D 16
+#endif
+#ifndef GNUFFT_TILE_2D
+#define GNUFFT_TILE_2D 32
+#endif
+constexpr uint32_t kTileChunk = GNUFFT_TILE_CHUNK;  // visits per work item
 constexpr uint32_t kTileFixedCost = 512;  // staging + store of a tile, in visits
 
-int tile_target(const Geom& g) { return g.dim == 3 ? 16 : 32; }
+int tile_target(const Geom& g) { return g.dim == 3 ? GNUFFT_TILE_3D : GNUFFT_TILE_2D; }
 
 int ensure_tiles(gnufft_plan p, hipStream_t s) {
   if (p->tiles_built) return GNUFFT_OK;
@@ -983,6 +992,9 @@ int gnufft_plan_set_gridding(gnufft_plan p, int kind) {
     return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_gridding: unknown kind");
   if (kind != p->gridding) p->generation++;
   p->gridding = kind;
+  // the tile tables of a plan with a trajectory are built here, so later
+  // calls (graph capture, the allocation-free spread) find them ready
+  if (kind == GNUFFT_GRIDDING_TILES && p->sdc_d) return ensure_tiles(p, nullptr);
   return GNUFFT_OK;
 }
 
@@ -1112,6 +1124,7 @@ int gnufft_plan_set_trajectory(gnufft_plan p, const float* traj, uint32_t n_samp
   p->M = M;
   p->g.M = (int)M;
   rc = build_chunks(p, s);
+  if (rc == GNUFFT_OK && use_tiles(p)) rc = ensure_tiles(p, s);
 done:
   dfree(traj_in);
   dfree(dens_in);
@@ -1244,6 +1257,7 @@ int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
     p->g.M = (int)M;
     rc = build_chunks(p, s);
   }
+  if (!rc && use_tiles(p)) rc = ensure_tiles(p, s);
   if (rc) free_traj(p);
   return rc;
 }

@@ -45,7 +45,13 @@
 
 namespace gnufft {
 
-constexpr int kTileThreads = 512;
+#ifndef GNUFFT_TILE_THREADS
+#define GNUFFT_TILE_THREADS 512
+#endif
+#ifndef GNUFFT_TILE_PER_CU
+#define GNUFFT_TILE_PER_CU 2
+#endif
+constexpr int kTileThreads = GNUFFT_TILE_THREADS;
 
 // Per-axis tile of a grid coordinate from the boundary table:
 // tiles [bound[t], bound[t+1]).
@@ -630,7 +636,8 @@ static int tile_pick(const Geom& g, const TileArgs& a, int nx, int per_cu, int* 
   return 0;
 }
 
-int tile_spread_blocks_per_cu(const Geom& g) { return g.dim == 3 ? 2 : 2; }
+// Workgroups per CU the LDS footprint is sized for.
+int tile_spread_blocks_per_cu(const Geom&) { return GNUFFT_TILE_PER_CU; }
 
 hipError_t launch_tile_spread(const Geom& g, const TileArgs& a, hipStream_t s) {
   if (a.n_coils == 0 || a.n_items == 0) return hipSuccess;
