@@ -60,20 +60,147 @@ def parse():
     ap.add_argument("--profile-copy-mb", type=int, default=0,
                     help="in profile mode, also copy this many MiB device-to-device "
                          "(known-size reference for HBM traffic counters)")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
+                    help="write the committed counter summaries of profiles/ from rocprofv3 "
+                         "--pmc counter_collection.csv files of --profile-steps 1 runs "
+                         "(no GPU needed)")
     return ap.parse_args()
 
 
+# sources whose kernels the traffic counters measured
+TRAFFIC_SOURCES = ["gpunufft_amd/csrc/hip/gridding.hip", "gpunufft_amd/csrc/hip/gnufft_internal.hpp"]
+
+
+def sources_sha256():
+    """SHA-256 over the kernel sources a counter summary was taken from."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def traffic_summary():
-    """HBM bytes per spread launch from the committed rocprofv3 counter runs."""
+    """HBM bytes per adjoint gridding launch from the committed rocprofv3
+    counter runs, or None when the summary was taken from other kernel
+    sources than the ones built here (its sources_sha256 stamp differs)."""
     p = os.path.join(ROOT, "profiles", "spread_koosh3d_128_mc8_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f)
+        t = json.load(f)
+    if t.get("sources_sha256") != sources_sha256():
+        return None
+    return t
+
+
+GRIDDING_KERNELS = ("spread", "fold", "interp")
+
+
+def summarize_counters(paths):
+    """Per-dispatch counter values of the gridding kernels from rocprofv3
+    --pmc runs of `bench.py --profile-steps 1 --profile-copy-mb 1024`.
+
+    HBM bytes: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what
+    the same counters read for the 1 GiB device-to-device copy of the same
+    run (the largest copy dispatch), so that the copy counts 1 GiB each way.
+    Writes profiles/spread_koosh3d_128_mc8_traffic.json (stamped with the
+    SHA-256 of the kernel sources) and profiles/gridding_koosh3d_128_mc8_counters.md."""
+    import csv
+    import collections
+
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"]
+                key = None
+                for k in GRIDDING_KERNELS:
+                    if name.startswith("void gnufft::%s" % k) and "batch" not in name:
+                        key = name.split("(")[0].replace("void gnufft::", "")
+                if "copyBuffer" in name:
+                    key = "copy"
+                if key is None:
+                    continue
+                vals[key][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    copy = vals.get("copy", {})
+    cal = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        if copy.get(c):
+            cal[c] = (1 << 30) / (max(copy[c]) * 1024.0)
+    kernels = {}
+    for k, cs in sorted(vals.items()):
+        if k == "copy":
+            continue
+        e = {c: sum(v) / len(v) for c, v in cs.items()}
+        if "FETCH_SIZE" in e and "FETCH_SIZE" in cal:
+            e["fetch_bytes"] = e["FETCH_SIZE"] * 1024.0 * cal["FETCH_SIZE"]
+        if "WRITE_SIZE" in e and "WRITE_SIZE" in cal:
+            e["write_bytes"] = e["WRITE_SIZE"] * 1024.0 * cal["WRITE_SIZE"]
+        kernels[k] = e
+    adj = [k for k in kernels if k.startswith("spread") or k.startswith("fold")]
+    hbm = sum(kernels[k].get("fetch_bytes", 0) + kernels[k].get("write_bytes", 0) for k in adj)
+    from gpunufft_amd import workloads as W
+    w = W.config(WORKLOAD)
+    d, M = w["traj"].shape
+    C = w["coils"]
+    Ng = int(round(w["osf"] * w["img_dims"][0])) ** 3
+    alg = 8 * M * C + 4 * d * M + 4 * M + 8 * Ng * C
+    out = {
+        "workload": "koosh3d 128^3 osf1.5 kw3 sw8, 16384x256 samples, 8 coils",
+        "sources_sha256": sources_sha256(),
+        "sources": TRAFFIC_SOURCES,
+        "commands": [
+            "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc <COUNTERS> "
+            "--output-format csv -d $R/gpurun_out/<dir> -o run -- python3 $R/bench.py "
+            "--profile-steps 1 --profile-copy-mb 1024   (one run per counter set)",
+            "python3 bench.py --summarize-counters $R/gpurun_out/<dir>/run_counter_collection.csv ...",
+        ],
+        "calibration": {"copy_bytes": 1 << 30, "scale": cal,
+                        "note": "FETCH_SIZE / WRITE_SIZE of a 1 GiB device copy mapped to 1 GiB"},
+        "kernels": kernels,
+        "adjoint_gridding_kernels": adj,
+        "hbm_bytes_per_launch": round(hbm) if hbm else None,
+        "algorithmic_bytes_per_launch": alg,
+        "traffic_over_algorithmic": round(hbm / alg, 3) if hbm else None,
+    }
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    if hbm:
+        with open(os.path.join(ROOT, "profiles", "spread_koosh3d_128_mc8_traffic.json"), "w") as f:
+            json.dump(out, f, indent=1)
+    lines = ["# Gridding kernel counters: koosh3d 128^3 osf 1.5 kw 3 sw 8, 16384x256 samples, "
+             "8 coils (one adjoint + forward step)", "",
+             "Per dispatch, averaged over dispatches; HBM bytes scaled by the 1 GiB copy "
+             "(calibration %s). Sources SHA-256 %s." % (json.dumps(cal), out["sources_sha256"]),
+             "", "Commands: " + " ; ".join(out["commands"]), ""]
+    cols = ["fetch_bytes", "write_bytes", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+            "SQ_WAIT_INST_LDS", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU"]
+    cols = [c for c in cols if any(c in e for e in kernels.values())]
+    lines.append("| kernel | " + " | ".join(cols) + " |")
+    lines.append("|---|" + "---|" * len(cols))
+    for k, e in kernels.items():
+        lines.append("| `%s` | " % k + " | ".join("%.4g" % e[c] if c in e else "-" for c in cols) + " |")
+    lines.append("")
+    for k, e in kernels.items():
+        if "SQ_WAIT_ANY" in e and "SQ_WAVE_CYCLES" in e:
+            lines.append("- `%s`: SQ_WAIT_ANY / SQ_WAVE_CYCLES = %.2f" % (k, e["SQ_WAIT_ANY"] / e["SQ_WAVE_CYCLES"]))
+        if "SQ_LDS_BANK_CONFLICT" in e and "SQ_INSTS_LDS" in e:
+            lines.append("- `%s`: SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS = %.2f cycles per LDS instruction"
+                         % (k, e["SQ_LDS_BANK_CONFLICT"] / e["SQ_INSTS_LDS"]))
+    if hbm:
+        lines.append("- adjoint gridding stage (%s): %.4g HBM bytes per launch = %.2fx the %d "
+                     "algorithmic bytes" % (", ".join(adj), hbm, hbm / alg, alg))
+    with open(os.path.join(ROOT, "profiles", "gridding_koosh3d_128_mc8_counters.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(json.dumps({"traffic": out["hbm_bytes_per_launch"], "kernels": list(kernels)}))
 
 
 def main():
     args = parse()
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
