@@ -24,8 +24,7 @@ OBJ = os.path.join(PKG, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["precompute.hip", "gridding.hip", "tiles.hip", "prepost.hip", "fft.hip", "capi.hip",
-               "cg.hip"]
+HIP_SOURCES = ["precompute.hip", "gridding.hip", "prepost.hip", "fft.hip", "capi.hip", "cg.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -21,7 +21,6 @@ OK = 0
 LOOKUP_NEAREST, LOOKUP_LINEAR = 0, 1
 CONVOLUTION, FFT, DEAPODIZATION, DENSITY_ESTIMATION = 0, 1, 2, 3
 FFT_FUSED, FFT_ROCFFT = 0, 1
-GRIDDING_SECTORS, GRIDDING_TILES = 0, 1
 HOST, DEVICE = 0, 1
 
 EXPORTS = [
@@ -34,7 +33,6 @@ EXPORTS = [
     "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
     "gnufft_batch_adjoint", "gnufft_batch_forward", "gnufft_cg_create", "gnufft_cg_destroy",
     "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft", "gnufft_check_precomputed",
-    "gnufft_plan_set_gridding", "gnufft_plan_get_gridding",
 ]
 
 T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
@@ -120,8 +118,6 @@ def load(path=LIB_PATH):
     L.gnufft_interp_sorted.argtypes = [vp, vp, vp, u32, vp]
     L.gnufft_plan_set_fft.argtypes = [vp, i32]
     L.gnufft_plan_get_fft.argtypes = [vp, ctypes.POINTER(i32)]
-    L.gnufft_plan_set_gridding.argtypes = [vp, i32]
-    L.gnufft_plan_get_gridding.argtypes = [vp, ctypes.POINTER(i32)]
     L.gnufft_plan_set_timing.argtypes = [vp, i32]
     L.gnufft_plan_get_timing.argtypes = [vp, ctypes.POINTER(Timing), i32]
     vpp = ctypes.POINTER(vp)
@@ -284,16 +280,6 @@ class Plan:
     def fft(self):
         k = ctypes.c_int()
         check(load().gnufft_plan_get_fft(self._h, ctypes.byref(k)))
-        return k.value
-
-    def set_gridding(self, kind):
-        """GRIDDING_SECTORS (sector sub-grids + fold) or GRIDDING_TILES (grid
-        tiles with neighbour visits) for the adjoint spread."""
-        check(load().gnufft_plan_set_gridding(self._h, int(kind)))
-
-    def gridding(self):
-        k = ctypes.c_int()
-        check(load().gnufft_plan_get_gridding(self._h, ctypes.byref(k)))
         return k.value
 
     def set_timing(self, enable=True):

@@ -24,6 +24,7 @@
 
 using namespace gnufft;
 
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -266,22 +267,6 @@ struct gnufft_plan_s {
   float2* fft_tw_d = nullptr;
   size_t fft_tw_off[3][3] = {};
   StageTimer timer;
-  // adjoint gridding kind (GNUFFT_GRIDDING_*) and the tile tables, built at
-  // the first adjoint that uses them
-  int gridding = GNUFFT_GRIDDING_SECTORS;
-  bool tiles_built = false;
-  TileGrid tiles{};
-  uint32_t* tile_bound_d = nullptr;
-  TileVisits visits;
-  uint4* titems_d = nullptr;
-  uint32_t n_titems = 0;
-  uint32_t t_xcd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  uint32_t t_max_range = 0;
-  uint4* ttasks_d = nullptr;
-  uint32_t n_ttasks = 0;
-  uint32_t n_tslots = 0;
-  float2* tpartial_d = nullptr;  // split-tile partials [slot][tile point][coil]
-  uint32_t tpartial_coils = 0;
   // bumped whenever device memory that a captured graph or a batch may
   // hold a pointer to is freed or replaced (trajectory tables, work grid,
   // scratch, sensitivities, image buffer) or the FFT kind changes
@@ -320,24 +305,12 @@ void free_traj(gnufft_plan p) {
   dfree(p->sector_chunks_d);
   dfree(p->reduce_d);
   dfree(p->scratch_d);
-  free_visits(&p->visits);
-  dfree(p->tile_bound_d);
-  dfree(p->titems_d);
-  dfree(p->ttasks_d);
-  dfree(p->tpartial_d);
-  p->tile_bound_d = nullptr;
-  p->titems_d = nullptr;
-  p->ttasks_d = nullptr;
-  p->tpartial_d = nullptr;
-  p->tpartial_coils = 0;
-  p->n_titems = p->n_ttasks = p->n_tslots = 0;
-  p->tiles_built = false;
-  p->reduce_d = nullptr;
-  p->n_reduce = 0;
   p->traj_sorted_d = p->dens_sorted_d = nullptr;
   p->idx_d = p->assigned_d = p->sdc_d = p->centers_d = nullptr;
   p->chunks_d = nullptr;
   p->sector_chunks_d = nullptr;
+  p->reduce_d = nullptr;
+  p->n_reduce = 0;
   p->scratch_d = nullptr;
   p->scratch_coils = 0;
   p->M = 0;
@@ -640,7 +613,9 @@ int build_chunks(gnufft_plan p, hipStream_t s) {
     const uint64_t mean = total ? ((uint64_t)p->M + total - 1) / total : 64;
     p->round_target = (uint32_t)std::min<uint64_t>(p->chunk, std::max<uint64_t>(64, (mean + 63) / 64 * 64));
   }
-  // sectors split over several chunks: their sub-grids are summed before the fold
+  // sectors split over several chunks: their sub-grids are summed before the
+  // fold (one workgroup per 256 words of each such sector, so the dense
+  // k-space centre spreads over the whole GPU)
   std::vector<uint2> tasks;
   for (uint32_t i = 0; i < total;) {
     uint32_t j = i + 1;
@@ -678,128 +653,6 @@ SpreadArgs spread_args(gnufft_plan p, const float2* data, float2* grid, uint32_t
   return a;
 }
 
-// Tile tables of the adjoint tile gridding: tile partition of the grid,
-// visit lists (own samples and neighbours within reach of each tile) and
-// the work items (tiles with more than kTileChunk visits are split into
-// parts whose partial tiles a reduction sums).
-#ifndef GNUFFT_TILE_CHUNK
-#define GNUFFT_TILE_CHUNK 4096
-#endif
-#ifndef GNUFFT_TILE_3D
-#define GNUFFT_TILE_3D 16
-#endif
-#ifndef GNUFFT_TILE_2D
-#define GNUFFT_TILE_2D 32
-#endif
-constexpr uint32_t kTileChunk = GNUFFT_TILE_CHUNK;  // visits per work item
-constexpr uint32_t kTileFixedCost = 512;  // staging + store of a tile, in visits
-
-int tile_target(const Geom& g) { return g.dim == 3 ? GNUFFT_TILE_3D : GNUFFT_TILE_2D; }
-
-int ensure_tiles(gnufft_plan p, hipStream_t s) {
-  if (p->tiles_built) return GNUFFT_OK;
-  const Geom& g = p->g;
-  TileGrid t{};
-  std::vector<uint32_t> words;
-  size_t boff[3] = {0, 0, 0};
-  t.n_tiles = 1;
-  for (int a = 0; a < 3; a++) {
-    std::vector<uint32_t> b;
-    if (a < g.dim) {
-      tile_axis(g.G[a], tile_target(g), &t.n[a], &t.Tm[a], &b);
-    } else {
-      t.n[a] = 1;
-      t.Tm[a] = 1;
-      b = {0u, 1u};
-    }
-    if (t.Tm[a] > 255) return fail(GNUFFT_ERR_UNSUPPORTED, "tile gridding: tile wider than 255 points");
-    boff[a] = words.size();
-    words.insert(words.end(), b.begin(), b.end());
-    t.n_tiles *= (uint32_t)t.n[a];
-  }
-  if ((uint64_t)t.n_tiles * 27ull >= (1ull << 32))
-    return fail(GNUFFT_ERR_UNSUPPORTED, "tile gridding: too many tiles");
-  int rc = copy_in((void**)&p->tile_bound_d, words.data(), words.size() * 4, GNUFFT_HOST, s);
-  if (rc) return rc;
-  for (int a = 0; a < 3; a++) t.bound[a] = p->tile_bound_d + boff[a];
-  hipError_t e = build_visits(g, t, p->traj_sorted_d, p->idx_d, p->dens_sorted_d, p->M, &p->visits, s);
-  if (e != hipSuccess)
-    return fail(GNUFFT_ERR_DEVICE, std::string("tile gridding: visit lists: ") + hipGetErrorString(e));
-  std::vector<uint32_t> vb((size_t)t.n_tiles + 1);
-  HIP_TRY(hipMemcpy(vb.data(), p->visits.vbeg, vb.size() * 4, hipMemcpyDeviceToHost));
-  std::vector<uint4> items, tasks;
-  std::vector<uint32_t> costs;
-  uint32_t slots = 0;
-  for (uint32_t ti = 0; ti < t.n_tiles; ti++) {
-    const uint32_t b = vb[ti], n = vb[ti + 1] - vb[ti];
-    const uint32_t parts = n <= kTileChunk ? 1u : (n + kTileChunk - 1) / kTileChunk;
-    if (parts == 1) {
-      items.push_back(make_uint4(ti, b, b + n, ~0u));
-      costs.push_back(n + kTileFixedCost);
-      continue;
-    }
-    const uint32_t per = (n + parts - 1) / parts;
-    tasks.push_back(make_uint4(ti, slots, parts, 0u));
-    for (uint32_t k = 0; k < parts; k++) {
-      const uint32_t kb = b + k * per, ke = std::min(b + n, kb + per);
-      items.push_back(make_uint4(ti, kb, ke, slots + k));
-      costs.push_back(ke - kb + kTileFixedCost);
-    }
-    slots += parts;
-  }
-  BatchList l{};
-  partition_items(costs.data(), (uint32_t)costs.size(), &l);
-  rc = copy_in((void**)&p->titems_d, items.data(), items.size() * sizeof(uint4), GNUFFT_HOST, s);
-  if (!rc && !tasks.empty())
-    rc = copy_in((void**)&p->ttasks_d, tasks.data(), tasks.size() * sizeof(uint4), GNUFFT_HOST, s);
-  if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(s));
-  p->tiles = t;
-  p->n_titems = (uint32_t)items.size();
-  p->n_ttasks = (uint32_t)tasks.size();
-  p->n_tslots = slots;
-  for (int k = 0; k < 9; k++) p->t_xcd[k] = l.xcd_begin[k];
-  p->t_max_range = l.max_range;
-  p->tiles_built = true;
-  return GNUFFT_OK;
-}
-
-int ensure_tile_partial(gnufft_plan p, uint32_t coils) {
-  if (p->n_tslots == 0 || p->tpartial_coils >= coils) return GNUFFT_OK;
-  p->generation++;
-  dfree(p->tpartial_d);
-  p->tpartial_d = nullptr;
-  p->tpartial_coils = 0;
-  const size_t np = (size_t)p->tiles.Tm[0] * p->tiles.Tm[1] * p->tiles.Tm[2];
-  if (hipMalloc(&p->tpartial_d, (size_t)p->n_tslots * np * coils * sizeof(float2)) != hipSuccess)
-    return fail(GNUFFT_ERR_OUT_OF_MEMORY, "tile gridding: partial-tile scratch allocation");
-  p->tpartial_coils = coils;
-  return GNUFFT_OK;
-}
-
-TileArgs tile_args(gnufft_plan p, const float2* data, float2* grid, uint32_t n_coils, bool caller_order) {
-  TileArgs a{};
-  a.data = data;
-  a.coil_stride = p->M;
-  a.vcrd = p->visits.crd;
-  a.vsrc = caller_order ? p->visits.idx : p->visits.pos;
-  a.vdens = caller_order ? p->visits.dens : nullptr;
-  a.V = p->visits.V;
-  a.items = p->titems_d;
-  a.n_items = p->n_titems;
-  for (int k = 0; k < 9; k++) a.xcd_begin[k] = p->t_xcd[k];
-  a.max_range = p->t_max_range;
-  a.tasks = p->ttasks_d;
-  a.n_tasks = p->n_ttasks;
-  a.tiles = p->tiles;
-  a.lut = p->lut_d;
-  a.grid = grid;
-  a.partial = p->tpartial_d;
-  a.n_coils = (int)n_coils;
-  return a;
-}
-
-bool use_tiles(gnufft_plan p) { return p->gridding == GNUFFT_GRIDDING_TILES; }
 
 }  // namespace
 
@@ -986,23 +839,6 @@ int gnufft_plan_set_fft(gnufft_plan p, int kind) {
   return GNUFFT_OK;
 }
 
-int gnufft_plan_set_gridding(gnufft_plan p, int kind) {
-  if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_gridding: null plan");
-  if (kind != GNUFFT_GRIDDING_SECTORS && kind != GNUFFT_GRIDDING_TILES)
-    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_gridding: unknown kind");
-  if (kind != p->gridding) p->generation++;
-  p->gridding = kind;
-  // the tile tables of a plan with a trajectory are built here, so later
-  // calls (graph capture, the allocation-free spread) find them ready
-  if (kind == GNUFFT_GRIDDING_TILES && p->sdc_d) return ensure_tiles(p, nullptr);
-  return GNUFFT_OK;
-}
-
-int gnufft_plan_get_gridding(gnufft_plan p, int* kind) {
-  if (!p || !kind) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_gridding: null argument");
-  *kind = p->gridding;
-  return GNUFFT_OK;
-}
 
 int gnufft_plan_get_fft(gnufft_plan p, int* kind) {
   if (!p || !kind) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_fft: null argument");
@@ -1124,7 +960,6 @@ int gnufft_plan_set_trajectory(gnufft_plan p, const float* traj, uint32_t n_samp
   p->M = M;
   p->g.M = (int)M;
   rc = build_chunks(p, s);
-  if (rc == GNUFFT_OK && use_tiles(p)) rc = ensure_tiles(p, s);
 done:
   dfree(traj_in);
   dfree(dens_in);
@@ -1257,7 +1092,6 @@ int gnufft_plan_load_precomputed(gnufft_plan p, const float* traj_sorted,
     p->g.M = (int)M;
     rc = build_chunks(p, s);
   }
-  if (!rc && use_tiles(p)) rc = ensure_tiles(p, s);
   if (rc) free_traj(p);
   return rc;
 }
@@ -1354,18 +1188,10 @@ int gnufft_spread_sorted(gnufft_plan p, const void* data_sorted, void* grid, uin
   if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "spread_sorted: no trajectory set");
   hipStream_t s = (hipStream_t)stream;
   const uint32_t cc = coil_batch(p, n_coils);
-  int rc = use_tiles(p) ? ensure_tiles(p, s) : ensure_scratch(p, cc);
-  if (!rc && use_tiles(p)) rc = ensure_tile_partial(p, cc);
+  int rc = ensure_scratch(p, cc);
   if (rc) return rc;
   for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
     const uint32_t nc = std::min(cc, n_coils - c0);
-    if (use_tiles(p)) {
-      TileArgs a = tile_args(p, (const float2*)data_sorted + (size_t)c0 * p->M,
-                             (float2*)grid + (size_t)c0 * p->g.Ng, nc, false);
-      a.fold_shift = false;
-      TIMED(p, GNUFFT_T_SPREAD, s, launch_tile_spread(p->g, a, s));
-      continue;
-    }
     SpreadArgs a = spread_args(p, (const float2*)data_sorted + (size_t)c0 * p->M,
                                (float2*)grid + (size_t)c0 * p->g.Ng, nc);
     a.fold_shift = false;
@@ -1410,18 +1236,12 @@ int gnufft_adjoint(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils
   }
   {
     int rc = ensure_scratch(p, cc);
-    if (!rc && use_tiles(p)) rc = ensure_tiles(p, s);
-    if (!rc && use_tiles(p)) rc = ensure_tile_partial(p, cc);
     if (rc) return rc;
   }
   for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
     const uint32_t nc = std::min(cc, n_coils - c0);
     float2* grid = grid_out ? (float2*)out + (size_t)c0 * g.Ng : p->grid_d;
-    if (use_tiles(p)) {
-      TileArgs a = tile_args(p, (const float2*)kdata + (size_t)c0 * p->M, grid, nc, true);
-      a.fold_shift = !grid_out;
-      TIMED(p, GNUFFT_T_SPREAD, s, launch_tile_spread(g, a, s));
-    } else {
+    {
       SpreadArgs a = spread_args(p, (const float2*)kdata + (size_t)c0 * p->M, grid, nc);
       a.gather = p->idx_d;
       a.dens = p->dens_sorted_d;
@@ -1656,6 +1476,12 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
   auto batch_fused = [](const gnufft_plan q) { return q->fused && fused_fft_supported(q->g); };
   b->dim = g0.dim;
   b->nx = nx;
+  // kernel width 3 in every plan: the lane kernels (tap records in
+  // registers), flagged as nx = 3 for the batched launchers
+  bool lane = true;
+  for (uint32_t i = 0; i < n_plans; i++) lane = lane && lane_batch_ok(plans[i]->g);
+  if (lane) b->nx = 3;
+  const uint32_t grid_cb = lane ? (uint32_t)kBatchLaneCB : (uint32_t)kBatchCB;
   b->fused = false;
   b->fft_prime = false;
   for (uint32_t i = 0; i < n_plans; i++) {
@@ -1721,7 +1547,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
   for (uint32_t k = 0; k < n_plans && !rc; k++) {
     const gnufft_plan p = b->plans[k];
     const uint32_t C = b->coils[k];
-    const uint32_t cbs = (C + kBatchCB - 1) / kBatchCB;
+    const uint32_t cbs = (C + grid_cb - 1) / grid_cb;  // spread / interp coil blocks
     BatchOp op{};
     op.g = p->g;
     op.s = spread_args(p, nullptr, b->grid_d + grid_off[k], C);
@@ -1750,8 +1576,13 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
       rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: padded sector does not fit in LDS");
       break;
     }
-    b->spread_lds = std::max(b->spread_lds, spread_lds_bytes(p->g, kBatchCB, op.R_spread));
-    b->interp_lds = std::max(b->interp_lds, interp_lds_bytes(p->g, kBatchCB, op.R_interp));
+    if (lane) {
+      b->spread_lds = std::max(b->spread_lds, lane_lds_bytes(p->g, kBatchLaneCB));
+      b->interp_lds = std::max(b->interp_lds, lane_lds_bytes(p->g, kBatchLaneCB));
+    } else {
+      b->spread_lds = std::max(b->spread_lds, spread_lds_bytes(p->g, kBatchCB, op.R_spread));
+      b->interp_lds = std::max(b->interp_lds, interp_lds_bytes(p->g, kBatchCB, op.R_interp));
+    }
     const int ft = fold_threads(p->g, p->cover);
     if (ft == 0) {
       rc = fail(GNUFFT_ERR_UNSUPPORTED, "batch_create: too many fold candidates");
@@ -1788,8 +1619,9 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
       if (j - i > 1) rd.push_back(make_uint4(k, i, j - i, 0));
       i = j;
     }
+    const uint32_t fcbs = (C + kBatchCB - 1) / kBatchCB;  // fold coil blocks
     for (uint32_t sct = 0; sct < p->S; sct++)
-      for (uint32_t cb = 0; cb < cbs; cb++) {
+      for (uint32_t cb = 0; cb < fcbs; cb++) {
         fo.push_back(make_uint4(k, sct, cb, 0));
         fo_cost.push_back(1u);
       }

@@ -258,55 +258,6 @@ struct InterpArgs {
 };
 hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s);
 
-// tiles.hip: adjoint gridding on grid tiles with neighbour visits.
-// Tiles per axis and their boundaries (device): tile t of axis a covers
-// [bound[a][t], bound[a][t+1]); Tm[a] is the largest extent (LDS pitch).
-struct TileGrid {
-  int n[3];
-  int Tm[3];
-  const uint32_t* bound[3];
-  uint32_t n_tiles;
-};
-
-// Visits of every tile, grouped by tile (vbeg[t] .. vbeg[t+1]), own samples
-// first: sorted-sample position, caller index, coordinates [dim][V] and
-// density of each visit.
-struct TileVisits {
-  uint32_t V = 0;
-  uint32_t* vbeg = nullptr;
-  uint32_t* pos = nullptr;
-  uint32_t* idx = nullptr;
-  float* crd = nullptr;
-  float* dens = nullptr;
-};
-
-struct TileArgs {
-  const float2* data;     // [coil][...] samples
-  long long coil_stride;
-  const float* vcrd;      // visit coordinates [dim][V]
-  const uint32_t* vsrc;   // sample of each visit in `data` (caller index or sorted position)
-  const float* vdens;     // NULL: no density weighting
-  uint32_t V;
-  const uint4* items;     // (tile, visit begin, visit end, partial slot or ~0u)
-  uint32_t n_items;
-  uint32_t xcd_begin[9];
-  uint32_t max_range;
-  const uint4* tasks;     // split tiles: (tile, first slot, parts, 0)
-  uint32_t n_tasks;
-  TileGrid tiles;
-  const float* lut;
-  float2* grid;           // [coil][Ng]
-  float2* partial;        // [slot][tile point][coil]
-  int n_coils;
-  bool fold_shift;        // store at INVERSE-fftshifted positions
-};
-
-void tile_axis(int G, int target, int* n, int* Tm, std::vector<uint32_t>* bound);
-hipError_t build_visits(const Geom& g, const TileGrid& t, const float* crds_sorted,
-                        const uint32_t* idx, const float* dens_sorted, uint32_t M,
-                        TileVisits* out, hipStream_t s);
-void free_visits(TileVisits* v);
-hipError_t launch_tile_spread(const Geom& g, const TileArgs& a, hipStream_t s);
 
 // prepost.hip
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
@@ -373,17 +324,22 @@ void partition_items(const uint32_t* costs, uint32_t n, BatchList* l);
 // workgroups do not raise that count: the 8-coil sub-grid dominates the LDS
 // footprint, 20.7 KB at 2-D sw 16.)
 constexpr int kBatchCB = 8;
+constexpr int kBatchLaneCB = 4;  // coil block of the batched lane (kernel width 3) kernels
 constexpr int kBatchThreads = 512;
 constexpr int kBatchPerCU = 4;
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
-hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
-                               hipStream_t s);
 hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, int threads,
                              hipStream_t s);
 hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
 size_t spread_lds_bytes(const Geom& g, int cb, int R);
+// Batched lane kernels (launch_*_batch with nx == 3): usable for plan g;
+// their dynamic LDS for a coil block of cb.
+bool lane_batch_ok(const Geom& g);
+hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
+                               hipStream_t s);
+size_t lane_lds_bytes(const Geom& g, int cb);
 size_t interp_lds_bytes(const Geom& g, int cb, int R);
 int spread_round(const Geom& g, int cb, int chunk, size_t limit);
 int interp_round(const Geom& g, int cb, int chunk, size_t limit);

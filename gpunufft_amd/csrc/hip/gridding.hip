@@ -20,14 +20,21 @@
 // its trip counts are wave-uniform (kw, or kw+1 when any lane of the wave
 // needs it), so no lane idles through masked iterations.
 //
-// The spread accumulates the padded sector sub-grid in LDS in double
-// precision, two planes (re, im) laid out [point][coil]: on gfx950
-// ds_add_f64 sustains ~20x the rate of ds_add_f32, and double sums are at
-// least as accurate as the reference's float atomics. The slots of one wave
-// take samples that lie far apart in the round, so their taps rarely hit
-// the same LDS words.
+// The spread accumulates the padded sector sub-grid in LDS in 64-bit fixed
+// point, one word per (point, coil) laid out [point][coil] (re and im packed
+// as two 32-bit halves, ds_add_u64; see the Spread section): on gfx950 a
+// wave's ds_add_f32 costs ~190 clocks against ~11 for ds_add_u64 with
+// random addresses, and integer sums are exact and order-independent.
+//
+// Two kernel families: for kernel width 3 (every shipped workload) the
+// "lane" kernels form each sample's tap record in registers and share it
+// across the four lanes of a quad (one lane per coil) with DPP moves; other
+// widths use the round-based kernels below, which stage tap records in LDS
+// per round of samples.
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "gnufft_internal.hpp"
 
@@ -451,6 +458,303 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
   const uint32_t ci = work_chunk(a.work, blockIdx.x);
   if (ci == ~0u) return;
   spread_body<DIM, NX, CB>(g, a, ci, blockIdx.y, R);
+}
+
+// ---------------------------------------------------------------------------
+// Spread with tap records in registers (kernel widths with a compile-time
+// tap count KW; KW = 3 is instantiated).
+//
+// Each thread owns whole samples (sample j = round * NT + tid): it loads
+// their coordinates and density and forms their tap records (first tap per
+// axis, KW or KW+1 weights per axis) in registers. The four lanes of a quad
+// are one sample's coils (CB = 4; with CB = 2 two samples of two coils, with
+// CB = 1 four samples): in pass ps the quad works on the record of its lane
+// ps, fetched with DPP quad_perm moves (VALU, no LDS), and each lane adds
+// all KW^d taps of its coil. The CB coils of a point are consecutive LDS
+// words, so one atomic instruction touches 64 / CB points in runs of CB
+// words (as few bank conflicts as the [point][coil] layout allows), and the
+// samples of one instruction are four apart in the chunk.
+//
+// Nothing but the sub-grid, the kernel table and the per-axis k-space
+// positions lives in LDS; there are no LDS round trips of records or values
+// and three barriers per chunk (tables and L1 partials, scales, taps done).
+//
+// Per tap: two f64 fmas (the value pre-multiplied by the row weight wz*wy,
+// then by wx, rounded to an integer by the 1.5*2^52 trick), two integer
+// operations and one ds_add_u64 whose x offset is an immediate.
+//
+// Taps run KW per axis with no clamping. A window starts at most at Pmax-1
+// (the clamps of the reference's tap range), so the last of KW+1 taps is at
+// most index P+1; the sub-grid has a slack of 2P^2+2P+2 words (3-D; 2P+2 in
+// 2-D) past its end for such indices, whose taps carry weight zero (they lie
+// past the reference's last tap). A wave with any sample needing KW+1 taps
+// along an axis (a position exactly half-way between grid points) runs its
+// samples with KW+1 taps per axis.
+// ---------------------------------------------------------------------------
+struct LaneLayout {
+  int kpad, kpw, sub_n;
+  size_t kp_off, sub_off, bytes;
+  __host__ __device__ LaneLayout(const Geom& g, int kw, int cb) {
+    const int P = g.P;
+    const int PD = g.dim == 3 ? P * P * P : P * P;
+    const int slack = g.dim == 3 ? 2 * P * P + 2 * P + 2 : 2 * P + 2;
+    kpad = (g.K + 3) & ~3;
+    kpw = (3 * (P + kw + 1) + 3) & ~3;
+    sub_n = (PD + slack) * cb;
+    kp_off = (size_t)kpad * 4;
+    sub_off = (kp_off + (size_t)kpw * 4 + 15) & ~(size_t)15;
+    bytes = sub_off + (size_t)((sub_n + 1) & ~1) * 8;
+  }
+};
+
+// Fixed-point add of v * wt (both real parts pre-scaled) to one sub-grid word.
+__device__ inline void fx_add(unsigned long long* w, double wt, double vx, double vy) {
+  const uint32_t re = (uint32_t)__double_as_longlong(fma(wt, vx, kRoundMagic));
+  const uint32_t im = (uint32_t)__double_as_longlong(fma(wt, vy, kRoundMagic));
+  const uint32_t hi = im + (uint32_t)((int32_t)re >> 31);  // borrow of a negative re
+  atomicAdd(w, ((unsigned long long)hi << 32) | re);
+}
+
+// Taps of one sample and one coil: T per axis, row-major (z, y, x) as the
+// reference forms the weight (wz*wy)*wx. Indices are not clamped: taps past
+// the sample's window carry weight zero and land in the sub-grid or its
+// slack.
+template <int DIM, int T, int CB>
+__device__ inline void lane_taps(unsigned long long* sub, int P, const int* b, const float (*w)[4],
+                                 double vx, double vy, int cc) {
+  double wxd[T];
+#pragma unroll
+  for (int t = 0; t < T; t++) wxd[t] = (double)w[0][t];
+  unsigned long long* base = sub + ((b[2] * P + b[1]) * P + b[0]) * CB + cc;
+#pragma unroll
+  for (int tz = 0; tz < (DIM == 3 ? T : 1); tz++) {
+    const float wz = DIM == 3 ? w[2][tz] : 1.0f;
+#pragma unroll
+    for (int ty = 0; ty < T; ty++) {
+      const float wzy = DIM == 3 ? wz * w[1][ty] : w[1][ty];
+      const double rx = (double)wzy * vx, ry = (double)wzy * vy;
+      unsigned long long* row = base + (tz * P + ty) * P * CB;
+#pragma unroll
+      for (int tx = 0; tx < T; tx++) fx_add(row + tx * CB, wxd[tx], rx, ry);
+      // one row at a time: the scheduler would otherwise hoist the next
+      // rows' operands and spill
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Value of lane `src` of this lane's quad, for each of the four lanes of the
+// quad (DPP quad_perm: a VALU move, no LDS).
+template <int A, int B, int C, int D>
+__device__ inline uint32_t quad_get(uint32_t v) {
+  if (A == 0 && B == 1 && C == 2 && D == 3) return v;
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, A | (B << 2) | (C << 4) | (D << 6), 0xF, 0xF, false);
+}
+
+// Source lane pattern of pass p: lane qi of a quad works on the sample owned
+// by lane p * (4 / CB) + qi / CB, coil qi % CB.
+template <int CB, int PS>
+__device__ inline uint32_t quad_pass(uint32_t v) {
+  constexpr int SPQ = 4 / CB;
+  return quad_get<PS * SPQ + 0 / CB, PS * SPQ + 1 / CB, PS * SPQ + 2 / CB, PS * SPQ + 3 / CB>(v);
+}
+
+template <int CB>
+__device__ inline uint32_t quad_pass_rt(uint32_t v, int ps) {
+  // passes are unrolled; this maps the runtime index onto the compile-time
+  // patterns
+  switch (ps) {
+    case 0: return quad_pass<CB, 0>(v);
+    case 1: return CB > 1 ? quad_pass<CB, (CB > 1 ? 1 : 0)>(v) : v;
+    case 2: return CB > 2 ? quad_pass<CB, (CB > 2 ? 2 : 0)>(v) : v;
+    default: return CB > 3 ? quad_pass<CB, (CB > 3 ? 3 : 0)>(v) : v;
+  }
+}
+
+template <int DIM, int KW, int CB, int NT = kThreads>
+__device__ inline void spread_lane_body(const Geom& g, const SpreadArgs& a, uint32_t ci,
+                                        int coil_block) {
+  constexpr int NW = NT / 64;
+  constexpr int NX = KW + 1;
+  constexpr int RND = 2;        // rounds of NT owned samples: chunks hold at most 2 * NT
+  constexpr int PASSES = CB;    // passes per round: one per coil of a quad's sample
+  static_assert(CB == 1 || CB == 2 || CB == 4, "a sample's coils share one quad");
+  static_assert(NX <= 4, "tap records hold at most four taps per axis");
+
+  extern __shared__ float4 smem4[];
+  __shared__ float l1_part[NW][CB];
+  __shared__ double scale_s[CB];
+  __shared__ double inv_s[CB];
+  char* smem = reinterpret_cast<char*>(smem4);
+  const LaneLayout L(g, KW, CB);
+  float* lut = reinterpret_cast<float*>(smem);
+  float* kp = reinterpret_cast<float*>(smem + L.kp_off);
+  unsigned long long* sub = reinterpret_cast<unsigned long long*>(smem + L.sub_off);
+  const int P = g.P;
+  const int Pmax = g.Pmax;
+  const int KPS = P + NX;
+  const int PD = DIM == 3 ? P * P * P : P * P;
+
+  const Chunk ch = a.work.chunks[ci];
+  const int n = (int)(ch.end - ch.begin);
+  const int coil0 = coil_block * CB;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int qc = (lane & 3) % CB;           // coil (in the block) of this lane
+  const bool coil_ok = coil0 + qc < a.n_coils;
+
+  // owned samples: j = r * NT + tid (a quad owns four consecutive samples,
+  // so one atomic's lanes take samples four apart; waves past the chunk's
+  // end own none and skip the taps)
+  float p[RND][3];
+  float sq[RND];
+  uint32_t src[RND];
+  bool live[RND];
+#pragma unroll
+  for (int r = 0; r < RND; r++) {
+    const int j = r * NT + tid;
+    live[r] = j < n;
+    p[r][0] = p[r][1] = p[r][2] = 0.0f;
+    sq[r] = 0.0f;
+    src[r] = 0u;
+    if (!live[r]) continue;
+    const uint32_t s = ch.begin + (uint32_t)j;
+    load_coord<DIM>(a.crds, g.M, s, p[r]);
+    sq[r] = a.dens ? sqrtf(a.dens[s]) : 1.0f;
+    src[r] = a.gather ? a.gather[s] : s;
+  }
+  // values: in pass ps this lane works on the sample of quad lane
+  // ps * (4 / CB) + qi / CB, coil qc
+  // (value times the square root of the density weight, in float)
+  float2 v[RND][PASSES];
+  bool livep[RND][PASSES];
+#pragma unroll
+  for (int r = 0; r < RND; r++) {
+#pragma unroll
+    for (int ps = 0; ps < PASSES; ps++) {
+      const uint32_t so = quad_pass_rt<CB>(src[r], ps);
+      livep[r][ps] = quad_pass_rt<CB>(live[r] ? 1u : 0u, ps) != 0u;
+      v[r][ps] = make_float2(0.0f, 0.0f);
+      if (livep[r][ps] && coil_ok) v[r][ps] = a.data[(size_t)(coil0 + qc) * a.coil_stride + so];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RND; r++)
+#pragma unroll
+    for (int ps = 0; ps < PASSES; ps++) {
+      const float sqo = __uint_as_float(quad_pass_rt<CB>(__float_as_uint(sq[r]), ps));
+      v[r][ps].x *= sqo;
+      v[r][ps].y *= sqo;
+    }
+
+  int c3[3];
+  load_center<DIM>(g, ch.sector, c3);
+  copy_to_lds<NT, 4>(lut, a.lut, g.K);
+  fill_kpos<DIM, NT>(g, c3, kp, KPS);
+  {
+    ulonglong2* sub2 = reinterpret_cast<ulonglong2*>(sub);
+    for (int i = tid; i < (L.sub_n + 1) / 2; i += NT) sub2[i] = make_ulonglong2(0ull, 0ull);
+  }
+
+  // per-coil L1 of the chunk (density-weighted): the fixed-point scale
+  {
+    float l1 = 0.0f;
+#pragma unroll
+    for (int r = 0; r < RND; r++)
+#pragma unroll
+      for (int ps = 0; ps < PASSES; ps++)
+        l1 += fmaxf(fabsf(v[r][ps].x), fabsf(v[r][ps].y));
+    // sum over the lanes of the same coil (lane bits from log2(CB) up)
+#pragma unroll
+    for (int o = CB; o < 64; o <<= 1) l1 += __shfl_xor(l1, o);
+    if (lane < CB) l1_part[wave][lane] = l1;
+  }
+  __syncthreads();  // tables, zeroed sub-grid and L1 partials ready
+  if (tid < CB) {
+    double t = 0.0;
+    for (int w = 0; w < NW; w++) t += (double)l1_part[w][tid];
+    const double sc = t > 0.0 ? kFixedOne / t : 0.0;
+    scale_s[tid] = sc;
+    inv_s[tid] = sc > 0.0 ? 1.0 / sc : 0.0;
+  }
+  __syncthreads();  // scales ready
+  const double scale = scale_s[qc];
+
+#pragma unroll
+  for (int r = 0; r < RND; r++) {
+    if (r * NT >= n) break;  // workgroup-uniform: no thread owns a sample of round r
+    if (r * NT + wave * 64 >= n) continue;  // wave-uniform: this wave owns none
+    // own tap record: first tap per axis (packed, 8 bits each) and NX
+    // weights per axis (zero past the sample's taps or failing the radius
+    // test)
+    uint32_t bp = 0;
+    float w[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) w[d][t] = 0.0f;
+    bool wide = false;
+    if (live[r]) {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const float u = map_kspace_to_grid(p[r][d], g.G[d], c3[d], g.off);
+        int lo, hi;
+        tap_range(u, Pmax, g.kradius, &lo, &hi);
+        bp |= (uint32_t)lo << (8 * d);
+        wide = wide || hi - lo + 1 > KW;
+#pragma unroll
+        for (int t = 0; t < NX; t++) {
+          float wt = 0.0f;
+          if (lo + t <= hi && axis_weight_kp(g, lut, d, kp[d * KPS + lo + t], p[r][d], &wt))
+            w[d][t] = wt;
+        }
+      }
+    }
+    const bool any_wide = __ballot(wide) != 0ull;
+#pragma unroll
+    for (int ps = 0; ps < PASSES; ps++) {
+      // the worked sample's record from its owner lane (every lane takes
+      // part in the moves)
+      const uint32_t bo = quad_pass_rt<CB>(bp, ps);
+      float wo[3][4];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          wo[d][t] = (d < DIM && t < (any_wide ? NX : KW))
+                         ? __uint_as_float(quad_pass_rt<CB>(__float_as_uint(w[d][t]), ps))
+                         : 0.0f;
+      if (!livep[r][ps] || !coil_ok) continue;
+      const int b[3] = {(int)(bo & 255u), (int)((bo >> 8) & 255u), (int)((bo >> 16) & 255u)};
+      const double vx = (double)v[r][ps].x * scale, vy = (double)v[r][ps].y * scale;
+      if (any_wide)
+        lane_taps<DIM, NX, CB>(sub, P, b, wo, vx, vy, qc);
+      else
+        lane_taps<DIM, KW, CB>(sub, P, b, wo, vx, vy, qc);
+    }
+  }
+  __syncthreads();
+
+  // store the chunk's sub-grid (as spread_body)
+  float2* dst = a.scratch + (size_t)ci * a.n_coils * PD + coil0;
+  for (int e = tid; e < PD * CB; e += NT) {
+    const int pt = e / CB;
+    const int cc = e % CB;
+    if (coil0 + cc >= a.n_coils) continue;
+    const unsigned long long u = sub[e];
+    const int32_t re = (int32_t)(uint32_t)u;
+    const int32_t im = (int32_t)(uint32_t)(u >> 32) - (re >> 31);
+    const double inv = inv_s[cc];
+    dst[(size_t)pt * a.n_coils + cc] = make_float2((float)(re * inv), (float)(im * inv));
+  }
+}
+
+template <int DIM, int KW, int CB>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) spread_lane_kernel(Geom g, SpreadArgs a) {
+  const uint32_t ci = work_chunk(a.work, blockIdx.x);
+  if (ci == ~0u) return;
+  spread_lane_body<DIM, KW, CB>(g, a, ci, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -982,6 +1286,190 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(Geom g, InterpArgs a, 
 }
 
 // ---------------------------------------------------------------------------
+// Interp with tap records in registers (kernel width KW = 3), the forward
+// twin of spread_lane_body: every thread forms the record of the samples it
+// owns, the lanes of a quad share them with DPP moves, and each lane sums
+// the KW^d taps of one (sample, coil) from the staged padded sector. The
+// samples of a quad are consecutive (the scattered output stores of a wave
+// cover runs of neighbouring samples). No LDS round trip of records and one
+// barrier after the staging.
+// ---------------------------------------------------------------------------
+template <int DIM, int T, int CB>
+__device__ inline void lane_gather(const float2* sub, int P, const int* b, const float (*w)[4],
+                                   int cc, float* ax, float* ay) {
+  const float2* base = sub + ((b[2] * P + b[1]) * P + b[0]) * CB + cc;
+  float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+  for (int tz = 0; tz < (DIM == 3 ? T : 1); tz++) {
+    const float wz = DIM == 3 ? w[2][tz] : 1.0f;
+#pragma unroll
+    for (int ty = 0; ty < T; ty++) {
+      const float wzy = DIM == 3 ? wz * w[1][ty] : w[1][ty];
+      const float2* row = base + (tz * P + ty) * P * CB;
+      float2 v[T];
+#pragma unroll
+      for (int tx = 0; tx < T; tx++) v[tx] = row[tx * CB];
+#pragma unroll
+      for (int tx = 0; tx < T; tx++) {
+        const float val = wzy * w[0][tx];
+        sx += v[tx].x * val;
+        sy += v[tx].y * val;
+      }
+    }
+  }
+  *ax = sx;
+  *ay = sy;
+}
+
+template <int DIM, int KW, int CB, int NT = kThreads>
+__device__ inline void interp_lane_body(const Geom& g, const InterpArgs& a, uint32_t ci,
+                                        int coil_block) {
+  constexpr int NX = KW + 1;
+  constexpr int RND = 2;
+  constexpr int PASSES = CB;
+  static_assert(CB == 1 || CB == 2 || CB == 4, "a sample's coils share one quad");
+
+  extern __shared__ float4 smem4[];
+  char* smem = reinterpret_cast<char*>(smem4);
+  const LaneLayout L(g, KW, CB);
+  float* lut = reinterpret_cast<float*>(smem);
+  float* kp = reinterpret_cast<float*>(smem + L.kp_off);
+  float2* sub = reinterpret_cast<float2*>(smem + L.sub_off);
+  const int P = g.P;
+  const int Pmax = g.Pmax;
+  const int KPS = P + NX;
+  const int PD = DIM == 3 ? P * P * P : P * P;
+
+  const Chunk ch = a.work.chunks[ci];
+  const int n = (int)(ch.end - ch.begin);
+  const int coil0 = coil_block * CB;
+  const int tid = threadIdx.x;
+  const int qc = (tid & 3) % CB;
+  const bool coil_ok = coil0 + qc < a.n_coils;
+  int c3[3];
+  load_center<DIM>(g, ch.sector, c3);
+
+  // owned samples: j = r * NT + tid (a quad owns four consecutive samples)
+  float p[RND][3];
+  bool live[RND];
+#pragma unroll
+  for (int r = 0; r < RND; r++) {
+    const int j = r * NT + tid;
+    live[r] = j < n;
+    p[r][0] = p[r][1] = p[r][2] = 0.0f;
+    if (live[r]) load_coord<DIM>(a.crds, g.M, ch.begin + (uint32_t)j, p[r]);
+  }
+
+  copy_to_lds<NT, 4>(lut, a.lut, g.K);
+  fill_kpos<DIM, NT>(g, c3, kp, KPS);
+  // Stage the padded sector as [point][coil] (periodic wrap and the FORWARD
+  // fftshift folded into the grid index), zeros in the slack past it (its
+  // words meet only zero weights, but must not hold NaNs). Four loads per
+  // thread in flight before their LDS writes.
+  {
+    constexpr int U = 4;
+    const int cc = tid % CB;
+    const bool cin = coil0 + cc < a.n_coils;
+    const float2* gbase = a.grid + (size_t)(cin ? coil0 + cc : 0) * g.Ng;
+    int q0[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      q0[d] = d < DIM ? c3[d] - g.off + (a.fold_shift ? g.shift_fwd[d] : 0) : 0;
+    for (int e0 = tid; e0 < L.sub_n; e0 += U * NT) {
+      float2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int pt = (e0 + u * NT) / CB;
+        v[u] = make_float2(0.0f, 0.0f);
+        if (pt < PD && cin) {
+          const int yz = (int)udiv((uint32_t)pt, g.mP);
+          const int x = pt - yz * P;
+          const int z = DIM == 3 ? (int)udiv((uint32_t)pt, g.mP2) : 0;
+          const int y = yz - z * P;
+          const int gx = wrap(q0[0] + x, g.G[0]);
+          const int gy = wrap(q0[1] + y, g.G[1]);
+          const int gz = DIM == 3 ? wrap(q0[2] + z, g.G[2]) : 0;
+          v[u] = gbase[(size_t)gx + (size_t)g.G[0] * ((size_t)gy + (size_t)g.G[1] * gz)];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (e0 + u * NT < L.sub_n) sub[e0 + u * NT] = v[u];
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int r = 0; r < RND; r++) {
+    if (r * NT >= n) break;  // workgroup-uniform
+    if (r * NT + (tid & ~63) >= n) continue;  // wave-uniform: this wave owns none
+    uint32_t bp = 0;
+    float w[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) w[d][t] = 0.0f;
+    bool wide = false;
+    if (live[r]) {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const float u = map_kspace_to_grid(p[r][d], g.G[d], c3[d], g.off);
+        int lo, hi;
+        tap_range(u, Pmax, g.kradius, &lo, &hi);
+        bp |= (uint32_t)lo << (8 * d);
+        wide = wide || hi - lo + 1 > KW;
+#pragma unroll
+        for (int t = 0; t < NX; t++) {
+          float wt = 0.0f;
+          if (lo + t <= hi && axis_weight_kp(g, lut, d, kp[d * KPS + lo + t], p[r][d], &wt))
+            w[d][t] = wt;
+        }
+      }
+    }
+    const bool any_wide = __ballot(wide) != 0ull;
+    const uint32_t lv = live[r] ? 1u : 0u;
+#pragma unroll
+    for (int ps = 0; ps < PASSES; ps++) {
+      const uint32_t bo = quad_pass_rt<CB>(bp, ps);
+      const bool lo_live = quad_pass_rt<CB>(lv, ps) != 0u;
+      float wo[3][4];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          wo[d][t] = (d < DIM && t < (any_wide ? NX : KW))
+                         ? __uint_as_float(quad_pass_rt<CB>(__float_as_uint(w[d][t]), ps))
+                         : 0.0f;
+      // the worked sample: owner lane (tid & ~3) + ps * (4 / CB) + qi / CB
+      const int jo = r * NT + (tid & ~3) + ps * (4 / CB) + (tid & 3) / CB;
+      if (!lo_live || !coil_ok) continue;
+      const int b[3] = {(int)(bo & 255u), (int)((bo >> 8) & 255u), (int)((bo >> 16) & 255u)};
+      float ax, ay;
+      if (any_wide)
+        lane_gather<DIM, NX, CB>(sub, P, b, wo, qc, &ax, &ay);
+      else
+        lane_gather<DIM, KW, CB>(sub, P, b, wo, qc, &ax, &ay);
+      const uint32_t s = ch.begin + (uint32_t)jo;
+      float rx = ax * a.scale, ry = ay * a.scale;
+      if (a.dens) {
+        const float sq = sqrtf(a.dens[s]);
+        rx = rx * sq;
+        ry = ry * sq;
+      }
+      const uint32_t dst = a.scatter ? a.scatter[s] : s;
+      a.out[(size_t)(coil0 + qc) * a.coil_stride + dst] = make_float2(rx, ry);
+    }
+  }
+}
+
+template <int DIM, int KW, int CB>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) interp_lane_kernel(Geom g, InterpArgs a) {
+  const uint32_t ci = work_chunk(a.work, blockIdx.x);
+  if (ci == ~0u) return;
+  interp_lane_body<DIM, KW, CB>(g, a, ci, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 // Dynamic LDS a workgroup may use on the current device (160 KB per CU on
@@ -1066,6 +1554,79 @@ static hipError_t interp_launch(const Geom& g, const InterpArgs& a, int R, hipSt
   return hipGetLastError();
 }
 
+// Lane-per-sample spread: kernel width 3 with a padded sector of at least
+// kw+1 points (the slack bound), chunks of at most two samples per thread.
+static bool lane_spread_ok(const Geom& g, int chunk) {
+  static int mode = -1;  // GNUFFT_SPREAD=round forces the round-based kernel
+  if (mode < 0) {
+    const char* e = getenv("GNUFFT_SPREAD");
+    mode = e && strcmp(e, "round") == 0 ? 0 : 1;
+  }
+  return mode == 1 && g.kw == 3 && g.P >= 4 && chunk <= 2 * kThreads;
+}
+
+// Lane interp: 2-D only by default (at 128^3 osf 1.5 the 3-D lane interp
+// takes 0.81 ms against 0.65 ms for the round-based kernel; 2-D batches are
+// faster with it). GNUFFT_INTERP=round / =lane override, for tuning.
+static bool lane_interp_ok(const Geom& g, int chunk) {
+  static int mode = -1;
+  if (mode < 0) {
+    const char* e = getenv("GNUFFT_INTERP");
+    mode = e && strcmp(e, "round") == 0 ? 0 : (e && strcmp(e, "lane") == 0 ? 2 : 1);
+  }
+  const bool shape_ok = g.kw == 3 && g.P >= 4 && chunk <= 2 * kThreads;
+  return shape_ok && (mode == 2 || (mode == 1 && g.dim == 2));
+}
+
+template <int DIM, int CB>
+static hipError_t spread_lane_launch(const Geom& g, const SpreadArgs& a, hipStream_t s) {
+  const uint32_t n_blocks = work_blocks(a.work);
+  const int coil_blocks = (a.n_coils + CB - 1) / CB;
+  const size_t lds = LaneLayout(g, 3, CB).bytes;
+  hipError_t e = allow_lds(spread_lane_kernel<DIM, 3, CB>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((spread_lane_kernel<DIM, 3, CB>), dim3(n_blocks, coil_blocks),
+                     dim3(kThreads), lds, s, g, a);
+  return hipGetLastError();
+}
+
+// Largest coil block (not above the coil count) whose footprint lets
+// `per_cu` workgroups share a CU; 0 if none.
+static int lane_cb(const Geom& g, int n_coils, int per_cu) {
+  const size_t limit = lds_limit() / (size_t)per_cu - 1024;
+  for (int cb = 4; cb >= 1; cb /= 2) {
+    if (cb > 1 && cb / 2 >= n_coils) continue;
+    if (LaneLayout(g, 3, cb).bytes <= limit) return cb;
+  }
+  return 0;
+}
+
+// Workgroups per CU the lane kernels' coil block is sized for (default 3:
+// at 128^3 osf 1.5 three workgroups of 4 coils take the spread kernel
+// 1.25 -> 0.72 ms against four of 2; GNUFFT_LANE_PER_CU overrides, for
+// tuning).
+static int lane_per_cu() {
+  static int v = 0;
+  if (v == 0) {
+    const char* e = getenv("GNUFFT_LANE_PER_CU");
+    v = e ? atoi(e) : 3;
+    if (v < 1 || v > 8) v = 3;
+  }
+  return v;
+}
+
+template <int DIM>
+static hipError_t spread_lane(const Geom& g, const SpreadArgs& a, hipStream_t s) {
+  int cb = 0;
+  for (int per = lane_per_cu(); per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
+  switch (cb) {
+    case 4: return spread_lane_launch<DIM, 4>(g, a, s);
+    case 2: return spread_lane_launch<DIM, 2>(g, a, s);
+    case 1: return spread_lane_launch<DIM, 1>(g, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <int DIM, int NX>
 static hipError_t spread_nx(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s) {
   // Prefer a footprint that lets three workgroups share a CU (the spread
@@ -1120,7 +1681,10 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStrea
   if (a.n_coils == 0) return hipSuccess;
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
   hipError_t e = hipSuccess;
-  if (a.work.n_chunks > 0) {
+  if (a.work.n_chunks > 0 && lane_spread_ok(g, chunk)) {
+    e = g.dim == 3 ? spread_lane<3>(g, a, s) : spread_lane<2>(g, a, s);
+    if (e != hipSuccess) return e;
+  } else if (a.work.n_chunks > 0) {
     if (g.dim == 3)
       e = nx_of(g) == 4 ? spread_nx<3, 4>(g, a, chunk, s) : spread_nx<3, 8>(g, a, chunk, s);
     else
@@ -1158,9 +1722,34 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStrea
   return hipGetLastError();
 }
 
+template <int DIM, int CB>
+static hipError_t interp_lane_launch(const Geom& g, const InterpArgs& a, hipStream_t s) {
+  const uint32_t n_blocks = work_blocks(a.work);
+  const int coil_blocks = (a.n_coils + CB - 1) / CB;
+  const size_t lds = LaneLayout(g, 3, CB).bytes;
+  hipError_t e = allow_lds(interp_lane_kernel<DIM, 3, CB>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((interp_lane_kernel<DIM, 3, CB>), dim3(n_blocks, coil_blocks), dim3(kThreads),
+                     lds, s, g, a);
+  return hipGetLastError();
+}
+
+template <int DIM>
+static hipError_t interp_lane(const Geom& g, const InterpArgs& a, hipStream_t s) {
+  int cb = 0;
+  for (int per = lane_per_cu(); per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
+  switch (cb) {
+    case 4: return interp_lane_launch<DIM, 4>(g, a, s);
+    case 2: return interp_lane_launch<DIM, 2>(g, a, s);
+    case 1: return interp_lane_launch<DIM, 1>(g, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s) {
   if (a.work.n_chunks == 0 || a.n_coils == 0) return hipSuccess;
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
+  if (lane_interp_ok(g, chunk)) return g.dim == 3 ? interp_lane<3>(g, a, s) : interp_lane<2>(g, a, s);
   if (g.dim == 3) return nx_of(g) == 4 ? interp_nx<3, 4>(g, a, chunk, s) : interp_nx<3, 8>(g, a, chunk, s);
   return nx_of(g) == 4 ? interp_nx<2, 4>(g, a, chunk, s) : interp_nx<2, 8>(g, a, chunk, s);
 }
@@ -1198,16 +1787,6 @@ __global__ void __launch_bounds__(kFoldThreads) fold_batch_kernel(const BatchOp*
   fold_body<DIM, kBatchCB>(op.g, op.s, it.y, (int)it.z);
 }
 
-// task (operator, first chunk, chunk count, 0) per blockIdx.y
-__global__ void __launch_bounds__(256) reduce_batch_kernel(const BatchOp* __restrict__ ops,
-                                                           const uint4* __restrict__ tasks) {
-  const uint4 t = tasks[blockIdx.y];
-  const BatchOp& op = ops[t.x];
-  const size_t PD = op.g.dim == 3 ? (size_t)op.g.P * op.g.P * op.g.P : (size_t)op.g.P * op.g.P;
-  reduce_body(op.s.scratch, (size_t)op.s.n_coils * PD, t.y, t.z,
-              (size_t)blockIdx.x * 256 + threadIdx.x);
-}
-
 size_t lds_budget() { return lds_limit(); }
 
 size_t spread_lds_bytes(const Geom& g, int cb, int R) {
@@ -1236,8 +1815,47 @@ static hipError_t launch_items(K kernel, const BatchOp* ops, const BatchList& l,
   return hipGetLastError();
 }
 
+// task (operator, first chunk, chunk count, 0) per blockIdx.y
+__global__ void __launch_bounds__(256) reduce_batch_kernel(const BatchOp* __restrict__ ops,
+                                                           const uint4* __restrict__ tasks) {
+  const uint4 t = tasks[blockIdx.y];
+  const BatchOp& op = ops[t.x];
+  const size_t PD = op.g.dim == 3 ? (size_t)op.g.P * op.g.P * op.g.P : (size_t)op.g.P * op.g.P;
+  reduce_body(op.s.scratch, (size_t)op.s.n_coils * PD, t.y, t.z,
+              (size_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+// Batched lane kernels (kernel width 3 in every plan): coil blocks of
+// kBatchLaneCB, one sample's coils per quad.
+template <int DIM>
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6)))
+    spread_batch_lane_kernel(const BatchOp* __restrict__ ops, BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  spread_lane_body<DIM, 3, kBatchLaneCB, kBatchThreads>(op.g, op.s, it.y, (int)it.z);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6)))
+    interp_batch_lane_kernel(const BatchOp* __restrict__ ops, BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  interp_lane_body<DIM, 3, kBatchLaneCB, kBatchThreads>(op.g, op.i, it.y, (int)it.z);
+}
+
+bool lane_batch_ok(const Geom& g) {
+  return lane_spread_ok(g, 2 * kBatchThreads) && lane_interp_ok(g, 2 * kBatchThreads);
+}
+
+size_t lane_lds_bytes(const Geom& g, int cb) { return LaneLayout(g, 3, cb).bytes; }
+
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s) {
+  if (nx == 3)
+    return dim == 3 ? launch_items(spread_batch_lane_kernel<3>, ops, l, kBatchThreads, lds, s)
+                    : launch_items(spread_batch_lane_kernel<2>, ops, l, kBatchThreads, lds, s);
   if (dim == 3)
     return nx == 4 ? launch_items(spread_batch_kernel<3, 4>, ops, l, kBatchThreads, lds, s)
                    : launch_items(spread_batch_kernel<3, 8>, ops, l, kBatchThreads, lds, s);
@@ -1245,8 +1863,19 @@ hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchL
                  : launch_items(spread_batch_kernel<2, 8>, ops, l, kBatchThreads, lds, s);
 }
 
+hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
+                               hipStream_t s) {
+  if (l.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_batch_kernel, dim3((max_kstride + 255) / 256, l.n), dim3(256), 0, s,
+                     ops, l.items);
+  return hipGetLastError();
+}
+
 hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s) {
+  if (nx == 3)
+    return dim == 3 ? launch_items(interp_batch_lane_kernel<3>, ops, l, kBatchThreads, lds, s)
+                    : launch_items(interp_batch_lane_kernel<2>, ops, l, kBatchThreads, lds, s);
   if (dim == 3)
     return nx == 4 ? launch_items(interp_batch_kernel<3, 4>, ops, l, kBatchThreads, lds, s)
                    : launch_items(interp_batch_kernel<3, 8>, ops, l, kBatchThreads, lds, s);
@@ -1260,12 +1889,5 @@ hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, in
                   : launch_items(fold_batch_kernel<2>, ops, l, threads, 0, s);
 }
 
-hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
-                               hipStream_t s) {
-  if (l.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_batch_kernel, dim3((max_kstride + 255) / 256, l.n), dim3(256), 0, s,
-                     ops, l.items);
-  return hipGetLastError();
-}
 
 }  // namespace gnufft

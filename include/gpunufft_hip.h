@@ -124,22 +124,6 @@ enum { GNUFFT_FFT_FUSED = 0, GNUFFT_FFT_ROCFFT = 1 };
 int gnufft_plan_set_fft(gnufft_plan plan, int kind);
 int gnufft_plan_get_fft(gnufft_plan plan, int* kind);
 
-/* Adjoint gridding (spread) implementation of a plan. The reference
- * spreads each sector into a padded sub-grid and adds it into the grid with
- * global atomics (CUDA/src/gpu/atomic/atomic_gpuNUFFT_kernels.cu:302-617,
- * launchers :729-842; adjConvolution, CUDA/src/gpuNUFFT_operator.cpp:166-175).
- * GNUFFT_GRIDDING_SECTORS: per-sector-chunk padded sub-grids in LDS
- *   (fixed-point accumulation, bit-reproducible), stored to scratch and
- *   folded into the grid.
- * GNUFFT_GRIDDING_TILES: one workgroup per grid tile (16^3 / 32^2 points)
- *   accumulates the tile's own points from its own samples and the
- *   neighbouring samples within reach (visit lists built at the first call
- *   that needs them), float LDS atomics, each grid point stored once.
- * The forward (interpolation) path is the same for both. */
-enum { GNUFFT_GRIDDING_SECTORS = 0, GNUFFT_GRIDDING_TILES = 1 };
-int gnufft_plan_set_gridding(gnufft_plan plan, int kind);
-int gnufft_plan_get_gridding(gnufft_plan plan, int* kind);
-
 /* Sector precompute on the device: assignSectors + sort + sectorDataCount +
  * sorted trajectory/density (CUDA/src/gpuNUFFT_operator_factory.cpp:128-205,
  * :450-553; CUDA/src/gpu/precomp_kernels.cu:8-177). traj/dens live at

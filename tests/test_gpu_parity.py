@@ -128,72 +128,6 @@ def test_spread_interp_kernels(oracle, torch_dev, capi, case, C, lookup):
     _gridding_case(oracle, torch_dev, capi, tf(), img, osf, kw, sw, C, lookup, seed=C)
 
 
-def _tile_spread_case(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, seed):
-    """Tile gridding (GRIDDING_TILES) spread of sorted data against the oracle."""
-    rng = np.random.default_rng(seed)
-    pl = capi.Plan(img, kw, sw, osf, lookup)
-    pl.set_trajectory(dev(torch_dev, traj))
-    pl.set_gridding(capi.GRIDDING_TILES)
-    pre = oracle.precompute(traj, img, osf, kw, sw, lookup=lookup, order="stable")
-    gi = pre["info"]
-    lut = oracle.load_lut(osf, kw, lookup)
-    data = W.complex_normal(rng, (C, traj.shape[1]))
-    g_ref = oracle.spread(gi, lookup, lut, pre["traj_sorted"], data, pre["sector_data_count"],
-                          pre["sector_centers"])
-    grid = torch_dev.full((C,) + g_ref.shape[1:], float("nan"), dtype=torch_dev.complex64,
-                          device="cuda")
-    pl.spread_sorted(dev(torch_dev, data), grid, C)
-    got = host(grid)
-    assert np.all(np.isfinite(got)), "tile gridding left grid points unwritten"
-    assert oracle.rel_l2(got, g_ref) < TOL
-
-
-@pytest.mark.parametrize("lookup", [0, 1])
-@pytest.mark.parametrize("C", [1, 3, 8])
-@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
-def test_tile_spread(oracle, torch_dev, capi, case, C, lookup):
-    """Every point of the grid is written once (the output starts as NaN)
-    and equals the oracle's spread (rel-L2 <= 1e-5), on every geometry:
-    odd, anisotropic and tiny grids (one tile per axis, taps that wrap
-    onto the same tile), kernel widths 3-5, all sector widths."""
-    name, tf, img, osf, kw, sw = case
-    print("case", name, C, lookup, flush=True)
-    _tile_spread_case(oracle, torch_dev, capi, tf(), img, osf, kw, sw, C, lookup, seed=C + 17)
-
-
-def test_tile_spread_split_tiles(oracle, torch_dev, capi):
-    """Tiles with more visits than one workgroup takes (a dense k-space
-    centre, 3-D and 2-D): parts summed by the reduction."""
-    rng = np.random.default_rng(4)
-    traj = np.concatenate([rng.uniform(-0.02, 0.02, (3, 30000)),
-                           rng.uniform(-0.5, 0.5, (3, 3000))], axis=1).astype(np.float32)
-    _tile_spread_case(oracle, torch_dev, capi, traj, (40, 40, 40), 1.5, 3, 8, 4, 0, seed=5)
-    traj2 = np.concatenate([rng.uniform(-0.01, 0.01, (2, 40000)),
-                            rng.uniform(-0.5, 0.5, (2, 4000))], axis=1).astype(np.float32)
-    _tile_spread_case(oracle, torch_dev, capi, traj2, (96, 80), 2.0, 3, 8, 3, 1, seed=6)
-
-
-@pytest.mark.parametrize("case", [CASES[1], CASES[4], CASES[5], CASES[9]], ids=lambda c: c[0])
-def test_tile_adjoint_all_stages(oracle, torch_dev, capi, case):
-    """The full adjoint (caller order, density, sensitivities, every stop
-    stage) with tile gridding equals the oracle."""
-    name, tf, img, osf, kw, sw = case
-    traj = tf()
-    C = 4
-    dens = W.ramp_density(traj)
-    sens = W.gaussian_sens(img, C, seed=3)
-    pl, op = _op_pair(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, 0, dens, sens)
-    pl.set_gridding(capi.GRIDDING_TILES)
-    assert pl.gridding() == capi.GRIDDING_TILES
-    kd = W.complex_normal(np.random.default_rng(8), (C, traj.shape[1]))
-    ishape = oracle.img_shape(op.gi)
-    for stop, shape in ((oracle.CONVOLUTION, (C,) + oracle.grid_shape(op.gi)),
-                        (oracle.DEAPODIZATION, (1,) + ishape)):
-        out = torch_dev.empty(shape, dtype=torch_dev.complex64, device="cuda")
-        pl.adjoint(dev(torch_dev, kd), out, C, stop)
-        assert oracle.rel_l2(host(out), op.adjoint(kd, stop=stop)) < TOL, stop
-
-
 def test_heavy_sector_chunks(oracle, torch_dev, capi):
     """Thousands of samples in one sector (split over many workgroups)."""
     rng = np.random.default_rng(1)
@@ -321,9 +255,8 @@ def test_reference_fixtures_on_gpu(oracle, fixtures, torch_dev, capi):
     assert abs(host(out)[0, 0, 0].real - f["expect"]) < f["eps"]
 
 
-@pytest.mark.parametrize("gridding", [0, 1], ids=["sectors", "tiles"])
 @pytest.mark.parametrize("key", ["gpu_conv", "gpu_fft_stop"])
-def test_reference_gpu_known_answers(oracle, fixtures, torch_dev, capi, key, gridding):
+def test_reference_gpu_known_answers(oracle, fixtures, torch_dev, capi, key):
     """The reference's GPU convolution / FFT-stage known answers
     (gpuNUFFT_conv_tests.cpp, gpuNUFFT_2D_conv_tests.cpp,
     gpuNUFFT_fft_tests.cpp, transcribed in tests/golden) on the HIP path:
@@ -337,7 +270,6 @@ def test_reference_gpu_known_answers(oracle, fixtures, torch_dev, capi, key, gri
         data = np.array([complex(*v) for v in f["data"]], np.complex64)[None]
         pl = capi.Plan(tuple(f["img"]), f["kw"], f["sw"], f["osf"], 0)
         pl.set_trajectory(dev(torch_dev, traj))
-        pl.set_gridding(gridding)
         gi = oracle.make_info(tuple(f["img"]), f["osf"], f["kw"], f["sw"])
         shape = (1,) + (oracle.grid_shape(gi) if stop == oracle.CONVOLUTION else oracle.img_shape(gi))
         out = torch_dev.empty(shape, dtype=torch_dev.complex64, device="cuda")

@@ -47,9 +47,8 @@ BOUND = {"C1_radial2d_64": 0.012, "C2_golden2d_mc": 0.012, "C3_koosh3d_mc": 0.04
 _REF = {}
 
 
-@pytest.mark.parametrize("gridding", [0, 1], ids=["sectors", "tiles"])
 @pytest.mark.parametrize("name", NAMES)
-def test_ndft_error_per_workload(oracle, name, gridding):
+def test_ndft_error_per_workload(oracle, name):
     import torch
     from gpunufft_amd import capi
     traj, img, osf, sw, C, dens, use_sens = _analogue(name)
@@ -61,7 +60,6 @@ def test_ndft_error_per_workload(oracle, name, gridding):
     pl.set_trajectory(dev(traj), dev(dens))
     if use_sens:
         pl.set_sens(dev(sens), C)
-    pl.set_gridding(gridding)
     gi = oracle.make_info(img, osf, 3, sw)
     ishape = oracle.img_shape(gi)
     kd = W.complex_normal(rng, (C, M))
@@ -78,6 +76,5 @@ def test_ndft_error_per_workload(oracle, name, gridding):
     ref_a, ref_f = _REF[name]
     e_a = oracle.rel_l2(out.cpu().numpy(), ref_a)
     e_f = oracle.rel_l2(kout.cpu().numpy(), ref_f)
-    print("NDFT rel-L2 %s (%s): adjoint %.3e forward %.3e" % (
-        name, "tiles" if gridding else "sectors", e_a, e_f), flush=True)
+    print("NDFT rel-L2 %s: adjoint %.3e forward %.3e" % (name, e_a, e_f), flush=True)
     assert e_a < BOUND[name] and e_f < BOUND[name], (e_a, e_f)
