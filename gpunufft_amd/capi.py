@@ -33,6 +33,8 @@ EXPORTS = [
     "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
     "gnufft_batch_adjoint", "gnufft_batch_forward", "gnufft_cg_create", "gnufft_cg_destroy",
     "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft", "gnufft_check_precomputed",
+    "gnufft_adjoint_noalloc",
+    "gnufft_noalloc_capacity",
 ]
 
 T_SPREAD, T_INTERP, T_FFT, T_ZERO, T_PREPOST = range(5)
@@ -113,6 +115,8 @@ def load(path=LIB_PATH):
     L.gnufft_plan_get_tables.argtypes = [vp, u32p, u32p, u32p, f32p, f32p, f32p, f32p]
     L.gnufft_plan_get_assigned_sectors.argtypes = [vp, u32p]
     L.gnufft_adjoint.argtypes = [vp, vp, vp, u32, i32, vp]
+    L.gnufft_adjoint_noalloc.argtypes = [vp, vp, vp, u32, i32, vp]
+    L.gnufft_noalloc_capacity.restype = ctypes.c_size_t
     L.gnufft_forward.argtypes = [vp, vp, vp, u32, i32, vp]
     L.gnufft_spread_sorted.argtypes = [vp, vp, vp, u32, vp]
     L.gnufft_interp_sorted.argtypes = [vp, vp, vp, u32, vp]
@@ -260,6 +264,13 @@ class Plan:
     def adjoint(self, kdata, out, n_coils, stop=DEAPODIZATION, stream=None):
         check(load().gnufft_adjoint(self._h, _ptr_loc(kdata)[0], _ptr_loc(out)[0], n_coils,
                                     stop, _stream_ptr(stream)))
+        return out
+
+    def adjoint_noalloc(self, kdata, out, n_coils, stop=DEAPODIZATION, stream=None):
+        """gnufft_adjoint_noalloc: the adjoint with its work memory in the
+        library's fixed per-device area (no device allocation)."""
+        check(load().gnufft_adjoint_noalloc(self._h, _ptr_loc(kdata)[0], _ptr_loc(out)[0],
+                                            n_coils, stop, _stream_ptr(stream)))
         return out
 
     def forward(self, img, out, n_coils, stop=DEAPODIZATION, stream=None):

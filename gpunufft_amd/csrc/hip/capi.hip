@@ -24,6 +24,9 @@
 
 using namespace gnufft;
 
+#ifndef GNUFFT_NOALLOC_SCRATCH_BYTES
+#define GNUFFT_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#endif
 
 namespace {
 
@@ -1793,6 +1796,102 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
   }
   HIP_TRY(launch_interp_batch(b->dim, b->nx, b->ops_d, b->interp, b->interp_lds, s));
   return batch_mark_done(b, s);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint without allocation (no counterpart in the reference): the work
+// grid and the spread sub-grids live in a fixed per-device area, so the call
+// makes no hipMalloc / hipFree and no allocating library call. The area is
+// a file-scope device array; host code uses only the address that
+// hipGetSymbolAddress returns for the current device (the array's host-side
+// name is no device address).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocBytes = (size_t)GNUFFT_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_mu;
+std::map<int, unsigned char*> g_noalloc_area;  // device -> address of its `scratch`
+
+// Address of this device's area, looked up once per device (with that
+// device current, as the caller's calls are).
+int noalloc_area(unsigned char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  auto it = g_noalloc_area.find(dev);
+  if (it != g_noalloc_area.end()) {
+    *out = it->second;
+    return GNUFFT_OK;
+  }
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(GNUFFT_ERR_DEVICE, std::string("adjoint_noalloc: work area address: ") +
+                                       hipGetErrorString(e));
+  g_noalloc_area[dev] = static_cast<unsigned char*>(ptr);
+  *out = static_cast<unsigned char*>(ptr);
+  return GNUFFT_OK;
+}
+
+size_t round_up_256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+size_t gnufft_noalloc_capacity(void) { return kNoAllocBytes; }
+
+int gnufft_adjoint_noalloc(gnufft_plan p, const void* kdata, void* out, uint32_t n_coils, int stop,
+                           void* stream) {
+  if (!p || !out || (!kdata && p->M))
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint_noalloc: null argument");
+  if (!valid_stop(stop)) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint_noalloc: unknown stop stage");
+  if (!p->sdc_d) return fail(GNUFFT_ERR_STATE, "adjoint_noalloc: no trajectory set");
+  if (n_coils == 0) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint_noalloc: zero coils");
+  if (p->n_sens > 0 && p->n_sens != n_coils)
+    return fail(GNUFFT_ERR_INVALID_ARGUMENT, "adjoint_noalloc: coil count differs from sensitivity coils");
+  const bool grid_out = stop == GNUFFT_STOP_CONVOLUTION || stop == GNUFFT_STOP_DENSITY_ESTIMATION;
+  if (!grid_out && !p->fused)
+    return fail(GNUFFT_ERR_UNSUPPORTED,
+                "adjoint_noalloc: needs the fused FFT (rocFFT plans allocate work memory)");
+  const Geom& g = p->g;
+  // per coil: work grid (unless the caller's buffer is the grid) and the
+  // spread sub-grids / fused FFT intermediate
+  const size_t grid_b = grid_out ? 0 : (size_t)g.Ng * sizeof(float2);
+  const size_t scr_b = scratch_per_coil(p);
+  const size_t per = round_up_256(grid_b) + round_up_256(scr_b);
+  if (per == 0 || per > kNoAllocBytes)
+    return fail(GNUFFT_ERR_UNSUPPORTED,
+                "adjoint_noalloc: one coil needs " + std::to_string(per) +
+                    " bytes of work memory, more than the no-allocation area (" +
+                    std::to_string(kNoAllocBytes) + " bytes)");
+  unsigned char* area = nullptr;
+  int rc = noalloc_area(&area);
+  if (rc) return rc;
+  const uint32_t cc = (uint32_t)std::min<size_t>(n_coils, kNoAllocBytes / per);
+  const bool use_sens = p->n_sens > 0 && stop == GNUFFT_STOP_DEAPODIZATION;
+  hipStream_t s = (hipStream_t)stream;
+  for (uint32_t c0 = 0; c0 < n_coils; c0 += cc) {
+    const uint32_t nc = std::min(cc, n_coils - c0);
+    float2* grid = grid_out ? (float2*)out + (size_t)c0 * g.Ng : (float2*)area;
+    float2* scr = (float2*)(area + round_up_256(grid_b * nc));
+    SpreadArgs a = spread_args(p, (const float2*)kdata + (size_t)c0 * p->M, grid, nc);
+    a.scratch = scr;
+    a.gather = p->idx_d;
+    a.dens = p->dens_sorted_d;
+    a.fold_shift = !grid_out;
+    HIP_TRY(launch_spread(g, a, (int)p->round_target, s));
+    if (grid_out) continue;
+    float2* dst = use_sens ? (float2*)out : (float2*)out + (size_t)c0 * g.Ni;
+    HIP_TRY(fused_fft_adjoint(g, grid, scr, (int)nc, p->deapo_d,
+                              use_sens ? p->sens_d + (size_t)c0 * g.Ni : nullptr, dst, stop,
+                              use_sens && c0 > 0, p->fft_tw_d, p->fft_tw_off, s));
+  }
+  return GNUFFT_OK;
 }
 
 }  // extern "C"

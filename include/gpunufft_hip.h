@@ -189,6 +189,23 @@ int gnufft_plan_get_assigned_sectors(gnufft_plan plan, uint32_t* sectors_host);
 int gnufft_adjoint(gnufft_plan plan, const void* kdata, void* out, uint32_t n_coils, int stop,
                    void* stream);
 
+/* Adjoint that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same arguments
+ * and results as gnufft_adjoint; the plan (trajectory, sensitivities) must
+ * be set up beforehand. The work grid and the spread sub-grids live in a
+ * fixed area of gnufft_noalloc_capacity() bytes (64 MiB) per device, a
+ * static device array that each device holds until the process ends; coils
+ * run in batches that fit it. Every call on a device shares that device's
+ * area, so calls that overlap on one device race (serialise them, e.g. on
+ * one stream); each call writes every byte it reads, nothing needs
+ * clearing. GNUFFT_ERR_UNSUPPORTED when one coil's
+ * work memory exceeds the area, or when a stop that needs the FFT meets a
+ * plan without the fused FFT (rocFFT plans allocate). */
+int gnufft_adjoint_noalloc(gnufft_plan plan, const void* kdata, void* out, uint32_t n_coils,
+                           int stop, void* stream);
+size_t gnufft_noalloc_capacity(void);
+
 /* Forward: performForwardGpuNUFFT(GpuArray, GpuArray&, out),
  * CUDA/src/gpuNUFFT_operator.cpp:895-1059 (stage semantics of :1092-1268).
  * img device [1][img] with sensitivities (expanded to n_coils = sens coils)
