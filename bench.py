@@ -9,7 +9,7 @@ One step = one full adjoint (k-space [coil][sample] in the caller's order in
 HBM -> coil-combined image) plus one full forward (image -> k-space) through
 the C ABI, everything resident in HBM. value = 2*M*C / step time.
 
-roofline: the adjoint gridding stage (spread_kernel, reduce_chunks_kernel,
+roofline: the adjoint gridding stage (spread_lane_kernel, reduce_chunks_kernel,
 fold_kernel: samples in HBM -> grid in HBM), algorithmic bytes per launch
 8*M*C + 4*d*M + 4*M (density) + 8*Ng*C, over the stage's average time
 measured with HIP events on the launch stream (library stage timer).
@@ -64,6 +64,13 @@ def parse():
                     help="write the committed counter summaries of profiles/ from rocprofv3 "
                          "--pmc counter_collection.csv files of --profile-steps 1 runs "
                          "(no GPU needed)")
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="write profiles/koosh3d_128_mc8_kernel_trace.md from the "
+                         "kernel_stats.csv of a rocprofv3 --kernel-trace --stats run of "
+                         "--profile-steps 3 (no GPU needed)")
+    ap.add_argument("--sources-sha256", default=None,
+                    help="with --summarize-counters: the kernel-source hash the profiled run "
+                         "printed (default: the hash of the sources here)")
     return ap.parse_args()
 
 
@@ -95,10 +102,10 @@ def traffic_summary():
     return t
 
 
-GRIDDING_KERNELS = ("spread", "fold", "interp")
+GRIDDING_KERNELS = ("spread", "reduce", "fold", "interp")
 
 
-def summarize_counters(paths):
+def summarize_counters(paths, sha=None):
     """Per-dispatch counter values of the gridding kernels from rocprofv3
     --pmc runs of `bench.py --profile-steps 1 --profile-copy-mb 1024`.
 
@@ -115,10 +122,11 @@ def summarize_counters(paths):
         with open(path) as f:
             for r in csv.DictReader(f):
                 name = r["Kernel_Name"]
+                base = name[5:] if name.startswith("void ") else name
                 key = None
                 for k in GRIDDING_KERNELS:
-                    if name.startswith("void gnufft::%s" % k) and "batch" not in name:
-                        key = name.split("(")[0].replace("void gnufft::", "")
+                    if base.startswith("gnufft::%s" % k) and "batch" not in base:
+                        key = base.split("(")[0].replace("gnufft::", "")
                 if "copyBuffer" in name:
                     key = "copy"
                 if key is None:
@@ -139,7 +147,8 @@ def summarize_counters(paths):
         if "WRITE_SIZE" in e and "WRITE_SIZE" in cal:
             e["write_bytes"] = e["WRITE_SIZE"] * 1024.0 * cal["WRITE_SIZE"]
         kernels[k] = e
-    adj = [k for k in kernels if k.startswith("spread") or k.startswith("fold")]
+    adj = [k for k in kernels if k.split("<")[0] in ("spread_lane_kernel", "spread_kernel",
+                                                      "reduce_chunks_kernel", "fold_kernel")]
     hbm = sum(kernels[k].get("fetch_bytes", 0) + kernels[k].get("write_bytes", 0) for k in adj)
     from gpunufft_amd import workloads as W
     w = W.config(WORKLOAD)
@@ -149,7 +158,7 @@ def summarize_counters(paths):
     alg = 8 * M * C + 4 * d * M + 4 * M + 8 * Ng * C
     out = {
         "workload": "koosh3d 128^3 osf1.5 kw3 sw8, 16384x256 samples, 8 coils",
-        "sources_sha256": sources_sha256(),
+        "sources_sha256": sha or sources_sha256(),
         "sources": TRAFFIC_SOURCES,
         "commands": [
             "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc <COUNTERS> "
@@ -183,9 +192,9 @@ def summarize_counters(paths):
         lines.append("| `%s` | " % k + " | ".join("%.4g" % e[c] if c in e else "-" for c in cols) + " |")
     lines.append("")
     for k, e in kernels.items():
-        if "SQ_WAIT_ANY" in e and "SQ_WAVE_CYCLES" in e:
+        if "SQ_WAIT_ANY" in e and e.get("SQ_WAVE_CYCLES"):
             lines.append("- `%s`: SQ_WAIT_ANY / SQ_WAVE_CYCLES = %.2f" % (k, e["SQ_WAIT_ANY"] / e["SQ_WAVE_CYCLES"]))
-        if "SQ_LDS_BANK_CONFLICT" in e and "SQ_INSTS_LDS" in e:
+        if "SQ_LDS_BANK_CONFLICT" in e and e.get("SQ_INSTS_LDS"):
             lines.append("- `%s`: SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS = %.2f cycles per LDS instruction"
                          % (k, e["SQ_LDS_BANK_CONFLICT"] / e["SQ_INSTS_LDS"]))
     if hbm:
@@ -196,10 +205,43 @@ def summarize_counters(paths):
     print(json.dumps({"traffic": out["hbm_bytes_per_launch"], "kernels": list(kernels)}))
 
 
+def summarize_trace(path, sha=None):
+    """Kernel time table of a --profile-steps 3 kernel trace (three steps of
+    the headline workload after one plan set-up)."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    lines = ["# Kernel trace: koosh3d 128^3 osf 1.5 kw 3 sw 8, 16384x256 samples, 8 coils, "
+             "adjoint + forward x3", "",
+             "Command (one MI355X, from the repository root R):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+             "-d $R/gpurun_out/kt -o run -- python3 $R/bench.py --profile-steps 3",
+             "python3 bench.py --summarize-trace $R/gpurun_out/kt/run_kernel_stats.csv", "```", "",
+             "Kernel sources SHA-256 %s (bench.py sources_sha256)." % (sha or sources_sha256()), "",
+             "`--profile-steps 3` builds the plan (sector precompute, once) and runs three full "
+             "steps (adjoint: spread, chunk reduction, fold, fused FFT passes z, y, x; forward: "
+             "fused FFT passes x, y, z, interp). `__amd_rocclr_copyBuffer` is the host-to-device "
+             "upload of the synthetic workload.", "",
+             "| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
+    for r in rows:
+        name = r["Name"]
+        short = name.replace("(anonymous namespace)::", "").split("(")[0]
+        short = short.replace("void ", "").replace("gnufft::", "")
+        if "rocprim" in name:
+            short = "rocprim (sort/scan of the sector precompute)"
+        lines.append("| `%s` | %s | %.1f | %.1f | %.1f |" % (
+            short, r["Calls"], float(r["TotalDurationNs"]) / 1e3, float(r["AverageNs"]) / 1e3,
+            float(r["Percentage"])))
+    with open(os.path.join(ROOT, "profiles", "koosh3d_128_mc8_kernel_trace.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
     args = parse()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.sources_sha256)
+        return
     if args.summarize_counters:
-        summarize_counters(args.summarize_counters)
+        summarize_counters(args.summarize_counters, args.sources_sha256)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build has no multi-GPU path); "
@@ -253,7 +295,8 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"profile_steps": args.profile_steps,
-                          "copy_bytes": args.profile_copy_mb << 20}))
+                          "copy_bytes": args.profile_copy_mb << 20,
+                          "sources_sha256": sources_sha256()}))
         return
 
     for _ in range(args.warmup):
@@ -280,8 +323,8 @@ def main():
     achieved = spread_bytes / (spread_ms * 1e-3) / 1e9
     traffic = traffic_summary()
     roofline = {
-        "kernel": "adjoint gridding stage: spread_kernel + reduce_chunks_kernel + fold_kernel "
-                  "(3-D, 4-tap rows; coil blocks as launched)",
+        "kernel": "adjoint gridding stage: spread_lane_kernel + reduce_chunks_kernel + "
+                  "fold_kernel (3-D, kernel width 3, 4-coil blocks)",
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
