@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1559,6 +1560,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     op.s.dens = p->dens_sorted_d;
     op.s.fold_shift = true;
     op.s.n_reduce = 0;
+    op.s.sub_cb = lane ? kBatchLaneCB : kBatchCB;  // the batched spread's coil block
     op.i.grid = b->grid_d + grid_off[k];
     op.i.crds = p->traj_sorted_d;
     op.i.work = p->work;

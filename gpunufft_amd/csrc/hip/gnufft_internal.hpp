@@ -232,7 +232,8 @@ struct SpreadArgs {
   const float* crds;       // sorted SoA [dim][M]
   WorkList work;
   const float* lut;
-  float2* scratch;         // [chunk][padded point][coil] sub-grids (n_chunks*PD*n_coils)
+  float2* scratch;         // sub-grids [chunk][coil block][padded point][block width]
+  int sub_cb;              // coil block of that layout (the spread's; set by its launcher)
   const uint32_t* sector_chunks;  // [S+1] first chunk of each sector
   const uint2* reduce_tasks;      // (first chunk, count) of each multi-chunk sector
   uint32_t n_reduce;

@@ -50,6 +50,16 @@ __device__ inline uint32_t work_chunk(const WorkList& w, uint32_t bid) {
   return ci < w.xcd_begin[x + 1] ? ci : ~0u;
 }
 
+// Offset of (padded point pt, coil c) inside one chunk's sub-grids: coil
+// block c / sb (sb = the spread's coil block) holds its min(sb, C - block
+// start) coils of every point contiguously, so each spread workgroup writes
+// one contiguous run.
+__device__ inline size_t sub_offset(int c, int pt, int sb, int C, int PD) {
+  const int b0 = (c / sb) * sb;
+  const int w = min(sb, C - b0);
+  return (size_t)b0 * PD + (size_t)pt * w + (c - b0);
+}
+
 // Workgroups along x for a work list.
 static inline uint32_t work_blocks(const WorkList& w) { return 8u * (w.max_range ? w.max_range : 1u); }
 
@@ -440,16 +450,18 @@ __device__ inline void spread_body(const Geom& g, const SpreadArgs& a, uint32_t 
   // Store the chunk's sub-grid, [padded point][coil] like the LDS copy, with
   // plain coalesced stores; fold_kernel sums the sub-grids that overlap on
   // the grid (the coils of a point are one contiguous run for its reads).
-  float2* dst = a.scratch + (size_t)ci * a.n_coils * PD + coil0;
+  // this block's min(CB, C - coil0) coils of every point, one contiguous run
+  float2* dst = a.scratch + ((size_t)ci * a.n_coils + coil0) * PD;
+  const int wb = min(CB, a.n_coils - coil0);
   for (int e = tid; e < PD * CB; e += NT) {
     const int pt = e / CB;
     const int cc = e % CB;
-    if (coil0 + cc >= a.n_coils) continue;
+    if (cc >= wb) continue;
     const unsigned long long u = sub[e];
     const int32_t re = (int32_t)(uint32_t)u;
     const int32_t im = (int32_t)(uint32_t)(u >> 32) - (re >> 31);
     const double inv = inv_s[cc];
-    dst[(size_t)pt * a.n_coils + cc] = make_float2((float)(re * inv), (float)(im * inv));
+    dst[(size_t)pt * wb + cc] = make_float2((float)(re * inv), (float)(im * inv));
   }
 }
 
@@ -737,16 +749,18 @@ __device__ inline void spread_lane_body(const Geom& g, const SpreadArgs& a, uint
   __syncthreads();
 
   // store the chunk's sub-grid (as spread_body)
-  float2* dst = a.scratch + (size_t)ci * a.n_coils * PD + coil0;
+  // this block's min(CB, C - coil0) coils of every point, one contiguous run
+  float2* dst = a.scratch + ((size_t)ci * a.n_coils + coil0) * PD;
+  const int wb = min(CB, a.n_coils - coil0);
   for (int e = tid; e < PD * CB; e += NT) {
     const int pt = e / CB;
     const int cc = e % CB;
-    if (coil0 + cc >= a.n_coils) continue;
+    if (cc >= wb) continue;
     const unsigned long long u = sub[e];
     const int32_t re = (int32_t)(uint32_t)u;
     const int32_t im = (int32_t)(uint32_t)(u >> 32) - (re >> 31);
     const double inv = inv_s[cc];
-    dst[(size_t)pt * a.n_coils + cc] = make_float2((float)(re * inv), (float)(im * inv));
+    dst[(size_t)pt * wb + cc] = make_float2((float)(re * inv), (float)(im * inv));
   }
 }
 
@@ -860,10 +874,10 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
   }
   __syncthreads();
   const size_t kstride = (size_t)a.n_coils * PD;
-  const float2* scr = a.scratch + coil0;  // [chunk][padded point][coil]
-  // full block of an even coil count: every point's CB coils start on a
-  // 16-byte boundary
-  const bool vec = CB % 2 == 0 && coil0 + CB <= a.n_coils && a.n_coils % 2 == 0;
+  const int sb = a.sub_cb;
+  // full block, even coil count and even spread blocks: every coil pair of
+  // a point is one 16-byte word of one spread block
+  const bool vec = CB % 2 == 0 && coil0 + CB <= a.n_coils && a.n_coils % 2 == 0 && sb % 2 == 0;
   for (int base = 0; base < HP; base += nt * fold_pts(DIM)) {
 #pragma unroll
     for (int p = 0; p < fold_pts(DIM); p++) {
@@ -892,14 +906,14 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
             const uint32_t vx = cov[0][tx * kCovMax + ix];
             const int k = kofs[jyz * (int)ncx + (int)(vx & 255u)];
             if (k < 0) continue;
-            const float2* src =
-                scr + (size_t)k * kstride + (size_t)(pyz + (int)(vx >> 8)) * a.n_coils;
+            const float2* src = a.scratch + (size_t)k * kstride;
+            const int pt = pyz + (int)(vx >> 8);
             if (vec) {
               // the block's CB coils: CB/2 16-byte loads (16-byte aligned)
-              const float4* s4 = reinterpret_cast<const float4*>(src);
 #pragma unroll
               for (int c2 = 0; c2 < CB / 2; c2++) {
-                const float4 val = s4[c2];
+                const float4 val = *reinterpret_cast<const float4*>(
+                    src + sub_offset(coil0 + 2 * c2, pt, sb, a.n_coils, PD));
                 ax_[2 * c2] += val.x;
                 ay_[2 * c2] += val.y;
                 ax_[2 * c2 + 1] += val.z;
@@ -908,7 +922,7 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
             } else {
 #pragma unroll
               for (int c = 0; c < CB; c++) {
-                const float2 val = src[min(c, a.n_coils - 1 - coil0)];
+                const float2 val = src[sub_offset(min(coil0 + c, a.n_coils - 1), pt, sb, a.n_coils, PD)];
                 ax_[c] += val.x;
                 ay_[c] += val.y;
               }
@@ -1032,7 +1046,7 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
     for (int it = 0; it < n_items; it++) {
       const int4 v = items[it];
       const uint32_t rp = reps_of[it];
-      const float2* src = a.scratch + (size_t)v.w * kstride + coil0;  // [chunk][point][coil]
+      const float2* src = a.scratch + (size_t)v.w * kstride;
 #pragma unroll
       for (int p = 0; p < fold_pts(DIM); p++) {
         const int ix = wrap(q[p][0] - v.x, g.G[0]) + (int)(rp & 255u) * g.G[0];
@@ -1043,7 +1057,7 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
         const float m = in ? 1.0f : 0.0f;
 #pragma unroll
         for (int c = 0; c < CB; c++) {
-          const float2 val = src[(size_t)pt * a.n_coils + min(c, a.n_coils - 1 - coil0)];
+          const float2 val = src[sub_offset(min(coil0 + c, a.n_coils - 1), pt, a.sub_cb, a.n_coils, PD)];
           ax[p][c] += m * val.x;
           ay[p][c] += m * val.y;
         }
@@ -1616,9 +1630,11 @@ static int lane_per_cu() {
 }
 
 template <int DIM>
-static hipError_t spread_lane(const Geom& g, const SpreadArgs& a, hipStream_t s) {
+static hipError_t spread_lane(const Geom& g, SpreadArgs a, hipStream_t s, int* cb_out) {
   int cb = 0;
   for (int per = lane_per_cu(); per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
+  a.sub_cb = cb;
+  *cb_out = cb;
   switch (cb) {
     case 4: return spread_lane_launch<DIM, 4>(g, a, s);
     case 2: return spread_lane_launch<DIM, 2>(g, a, s);
@@ -1628,7 +1644,7 @@ static hipError_t spread_lane(const Geom& g, const SpreadArgs& a, hipStream_t s)
 }
 
 template <int DIM, int NX>
-static hipError_t spread_nx(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s) {
+static hipError_t spread_nx(const Geom& g, SpreadArgs a, int chunk, hipStream_t s, int* cb_out) {
   // Prefer a footprint that lets three workgroups share a CU (the spread
   // waits on memory most of the time; at 128^3 osf 1.5 three workgroups of
   // 4 coils beat two of 8 by 14 %), then two, then one.
@@ -1636,6 +1652,8 @@ static hipError_t spread_nx(const Geom& g, const SpreadArgs& a, int chunk, hipSt
   int cb = pick_cb<SpreadLayout>(g, a.n_coils, 8, chunk, lds_limit() / 3 - 1024, &R);
   if (cb == 0) cb = pick_cb<SpreadLayout>(g, a.n_coils, 8, chunk, lds_limit() / 2 - 1024, &R);
   if (cb == 0) cb = pick_cb<SpreadLayout>(g, a.n_coils, 8, chunk, lds_limit() - 1024, &R);
+  a.sub_cb = cb;
+  *cb_out = cb;
   switch (cb) {
     case 8: return spread_launch<DIM, NX, 8>(g, a, R, s);
     case 4: return spread_launch<DIM, NX, 4>(g, a, R, s);
@@ -1677,20 +1695,24 @@ int fold_threads(const Geom& g, const Cover& cover) {
   return threads;
 }
 
-hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s) {
-  if (a.n_coils == 0) return hipSuccess;
+hipError_t launch_spread(const Geom& g, const SpreadArgs& args, int chunk, hipStream_t s) {
+  if (args.n_coils == 0) return hipSuccess;
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
   hipError_t e = hipSuccess;
+  SpreadArgs a = args;
+  a.sub_cb = 1;  // set below to the spread's coil block (the sub-grid layout)
+  int cb = 1;
   if (a.work.n_chunks > 0 && lane_spread_ok(g, chunk)) {
-    e = g.dim == 3 ? spread_lane<3>(g, a, s) : spread_lane<2>(g, a, s);
+    e = g.dim == 3 ? spread_lane<3>(g, a, s, &cb) : spread_lane<2>(g, a, s, &cb);
     if (e != hipSuccess) return e;
   } else if (a.work.n_chunks > 0) {
     if (g.dim == 3)
-      e = nx_of(g) == 4 ? spread_nx<3, 4>(g, a, chunk, s) : spread_nx<3, 8>(g, a, chunk, s);
+      e = nx_of(g) == 4 ? spread_nx<3, 4>(g, a, chunk, s, &cb) : spread_nx<3, 8>(g, a, chunk, s, &cb);
     else
-      e = nx_of(g) == 4 ? spread_nx<2, 4>(g, a, chunk, s) : spread_nx<2, 8>(g, a, chunk, s);
+      e = nx_of(g) == 4 ? spread_nx<2, 4>(g, a, chunk, s, &cb) : spread_nx<2, 8>(g, a, chunk, s, &cb);
     if (e != hipSuccess) return e;
   }
+  a.sub_cb = cb;
   const int PDn = g.dim == 3 ? g.P * g.P * g.P : g.P * g.P;
   if (a.n_reduce > 0) {
     const size_t kstride = (size_t)a.n_coils * PDn;
@@ -1702,17 +1724,17 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStrea
   const uint32_t S = (uint32_t)g.S[0] * g.S[1] * g.S[2];
   const int threads = fold_threads(g, a.cover);
   if (threads == 0) return hipErrorInvalidValue;
-  const int cb = a.n_coils >= 8 ? 8 : a.n_coils >= 4 ? 4 : a.n_coils >= 2 ? 2 : 1;
-  const dim3 grid(S, (a.n_coils + cb - 1) / cb);
+  const int fcb = a.n_coils >= 8 ? 8 : a.n_coils >= 4 ? 4 : a.n_coils >= 2 ? 2 : 1;
+  const dim3 grid(S, (a.n_coils + fcb - 1) / fcb);
   if (g.dim == 3) {
-    switch (cb) {
+    switch (fcb) {
       case 8: hipLaunchKernelGGL((fold_kernel<3, 8>), grid, dim3(threads), 0, s, g, a); break;
       case 4: hipLaunchKernelGGL((fold_kernel<3, 4>), grid, dim3(threads), 0, s, g, a); break;
       case 2: hipLaunchKernelGGL((fold_kernel<3, 2>), grid, dim3(threads), 0, s, g, a); break;
       default: hipLaunchKernelGGL((fold_kernel<3, 1>), grid, dim3(threads), 0, s, g, a); break;
     }
   } else {
-    switch (cb) {
+    switch (fcb) {
       case 8: hipLaunchKernelGGL((fold_kernel<2, 8>), grid, dim3(threads), 0, s, g, a); break;
       case 4: hipLaunchKernelGGL((fold_kernel<2, 4>), grid, dim3(threads), 0, s, g, a); break;
       case 2: hipLaunchKernelGGL((fold_kernel<2, 2>), grid, dim3(threads), 0, s, g, a); break;
