@@ -128,6 +128,51 @@ def test_spread_interp_kernels(oracle, torch_dev, capi, case, C, lookup):
     _gridding_case(oracle, torch_dev, capi, tf(), img, osf, kw, sw, C, lookup, seed=C)
 
 
+def _kw1_tap_samples(traj, img, osf, kw, sw):
+    """Samples with kw + 1 taps along some axis (a position exactly half-way
+    between grid points): the tap range of the sample's own sector, in the
+    reference's float32 arithmetic (CUDA/inc/gpuNUFFT_utils.hpp:89-104,
+    194-199; E3 sector, CUDA/inc/precomp_utils.hpp:34-48)."""
+    P = sw + 2 * (kw // 2)
+    off, r = P // 2, np.float32(kw / 2.0)
+    wide = np.zeros(traj.shape[1], bool)
+    for a in range(traj.shape[0]):
+        G = int(math.floor(img[a] * osf))
+        Gf = np.float32(G)
+        pos = traj[a].astype(np.float32)
+        v = (pos * Gf).astype(np.float64) + 0.5 * float(Gf)
+        sec = np.clip(np.floor(np.round(v) / float(np.float32(sw))).astype(int), 0,
+                      int(math.ceil(G / sw)) - 1)
+        c = (sec * sw + sw // 2).astype(np.float32)
+        u = (pos * Gf + np.float32(0.5) * Gf - c + np.float32(off)).astype(np.float32)
+        wide |= (np.floor(u + r) - np.ceil(u - r) + 1) > kw
+    return int(wide.sum())
+
+
+@pytest.mark.parametrize("lookup", [0, 1])
+@pytest.mark.parametrize("C", [1, 4, 5])
+@pytest.mark.parametrize("case", [
+    ("radial2d_osf15", lambda: W.radial_2d(64, 128), (64, 64), 1.5, 3, 8),
+    ("axis3d_osf15", lambda: W.koosh_3d(6, 96), (32, 32, 32), 1.5, 3, 8)], ids=lambda c: c[0])
+def test_half_grid_positions(oracle, torch_dev, capi, case, C, lookup):
+    """Samples exactly half-way between grid points take kw + 1 taps along
+    that axis (the last with weight zero in nearest mode); the kernels run
+    such waves with four taps per axis. Spokes along the axes at osf 1.5
+    land on such positions every fourth sample: the case holds them (checked
+    first), and spread and interp equal the oracle (rel-L2 <= 1e-5)."""
+    name, tf, img, osf, kw, sw = case
+    traj = tf()
+    if name.startswith("axis3d"):
+        # spokes along x, y and z through the centre, plus the Fibonacci set
+        r = np.linspace(-0.5, 0.5, 96, endpoint=False).astype(np.float32)
+        z = np.zeros_like(r)
+        axes = np.concatenate([np.stack([r, z, z]), np.stack([z, r, z]), np.stack([z, z, r])], 1)
+        traj = np.concatenate([axes, traj], 1).astype(np.float32)
+    assert _kw1_tap_samples(traj, img, osf, kw, sw) >= 16
+    print("case", name, C, lookup, flush=True)
+    _gridding_case(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, seed=C + 3)
+
+
 def test_heavy_sector_chunks(oracle, torch_dev, capi):
     """Thousands of samples in one sector (split over many workgroups)."""
     rng = np.random.default_rng(1)
