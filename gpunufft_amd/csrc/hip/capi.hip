@@ -1368,6 +1368,7 @@ struct gnufft_batch_s {
   BatchList spread{}, interp{}, fold{}, reduce{}, post{}, pre{};
   size_t spread_lds = 0, interp_lds = 0;
   int fold_threads = 64;
+  bool fold_generic = false;  // some plan needs the fold's generic body
   uint32_t max_kstride = 0;
   float2* grid_d = nullptr;
   float2* scratch_d = nullptr;
@@ -1594,6 +1595,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
       break;
     }
     b->fold_threads = std::max(b->fold_threads, ft);
+    b->fold_generic = b->fold_generic || fold_generic(p->cover);
     b->max_kstride = std::max<uint32_t>(b->max_kstride, C * (uint32_t)padded_points(p->g));
     b->ops_h.push_back(op);
     if (b->fused) b->fft_desc_h.resize((k + 1) * fused_fft_desc_bytes());
@@ -1747,7 +1749,7 @@ int gnufft_batch_adjoint(gnufft_batch b, const void* const* kdata, void* const* 
   if (rc) return rc;
   HIP_TRY(launch_spread_batch(b->dim, b->nx, b->ops_d, b->spread, b->spread_lds, s));
   HIP_TRY(launch_reduce_batch(b->ops_d, b->reduce, b->max_kstride, s));
-  HIP_TRY(launch_fold_batch(b->dim, b->ops_d, b->fold, b->fold_threads, s));
+  HIP_TRY(launch_fold_batch(b->dim, b->ops_d, b->fold, b->fold_threads, b->fold_generic, s));
   if (b->fused) {
     for (int ps = 0; ps < fused_fft_pass_count(b->dim); ps++) {
       HIP_TRY(launch_fused_fft_batch(b->dim, true, ps, false, b->fft_prime, b->fft_desc_d,

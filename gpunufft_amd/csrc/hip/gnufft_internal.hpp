@@ -330,8 +330,11 @@ constexpr int kBatchThreads = 512;
 constexpr int kBatchPerCU = 4;
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
+// generic: some plan's cover exceeds the fold's per-coordinate tables
+// (fold_generic), so the kernel variant with the generic body runs
 hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, int threads,
-                             hipStream_t s);
+                             bool generic, hipStream_t s);
+bool fold_generic(const Cover& cover);
 hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
 size_t spread_lds_bytes(const Geom& g, int cb, int R);

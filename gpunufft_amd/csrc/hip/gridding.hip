@@ -100,7 +100,12 @@ __device__ inline bool batch_item(const BatchList& l, uint32_t bid, uint4* it) {
   const uint32_t x = bid & 7u;
   const uint32_t i = l.xcd_begin[x] + (bid >> 3);
   if (i >= l.xcd_begin[x + 1]) return false;
-  *it = l.items[i];
+  const uint4 v = l.items[i];
+  // workgroup-uniform: scalar registers, so the item's plan descriptor is
+  // read with scalar loads (as vector values every descriptor field would
+  // take VGPRs)
+  *it = make_uint4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                   __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
   return true;
 }
 
@@ -824,7 +829,7 @@ constexpr int kCovMax = 4;
 constexpr int kFoldMaxH = 256;  // sector width bound (P <= 255)
 
 template <int DIM, int CB>
-__device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+__device__ __forceinline__ void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
   __shared__ int kofs[kFoldCands];                   // chunk of candidate (jz, jy, jx), -1 empty
   __shared__ uint16_t cov[3][kFoldMaxH * kCovMax];   // candidate | padded index << 8
   __shared__ uint8_t ncov[3][kFoldMaxH];
@@ -947,7 +952,7 @@ __device__ inline void fold_body_cov(const Geom& g, const SpreadArgs& a, uint32_
 // Fold driven by the item list of all (candidate, periodic image) pairs;
 // every point reads every item (masked). For geometries beyond kCovMax.
 template <int DIM, int CB>
-__device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+__device__ __forceinline__ void fold_body_generic(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
   // item per non-empty candidate sector: padded origin per axis and the
   // chunk holding the sector's (reduced) sub-grid; rep: packed recurrence
   // offsets (P > G only)
@@ -1080,17 +1085,25 @@ __device__ inline void fold_body_generic(const Geom& g, const SpreadArgs& a, uin
   }
 }
 
-template <int DIM, int CB>
-__device__ inline void fold_body(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
-  if (a.cover.max_cov <= kCovMax)
-    fold_body_cov<DIM, CB>(g, a, s, coil_block);
-  else
-    fold_body_generic<DIM, CB>(g, a, s, coil_block);
+// GEN: also compiled with the generic body, for geometries beyond kCovMax.
+// The generic body's registers would otherwise size every fold: 2-D with 8
+// coils takes 63 VGPRs without it and 116 with it.
+template <int DIM, int CB, bool GEN>
+__device__ __forceinline__ void fold_body(const Geom& g, const SpreadArgs& a, uint32_t s, int coil_block) {
+  if constexpr (GEN) {
+    if (a.cover.max_cov > kCovMax) {
+      fold_body_generic<DIM, CB>(g, a, s, coil_block);
+      return;
+    }
+  }
+  fold_body_cov<DIM, CB>(g, a, s, coil_block);
 }
 
-template <int DIM, int CB>
+bool fold_generic(const Cover& cover) { return cover.max_cov > kCovMax; }
+
+template <int DIM, int CB, bool GEN>
 __global__ void __launch_bounds__(kFoldThreads) fold_kernel(Geom g, SpreadArgs a) {
-  fold_body<DIM, CB>(g, a, blockIdx.x, blockIdx.y);
+  fold_body<DIM, CB, GEN>(g, a, blockIdx.x, blockIdx.y);
 }
 
 void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], int max_len[3],
@@ -1148,6 +1161,40 @@ void build_cover(const Geom& g, std::vector<uint32_t>* words, size_t base[6], in
 // ---------------------------------------------------------------------------
 // Interp
 // ---------------------------------------------------------------------------
+// Interp taps of one (sample, coil): ny x nz rows (wave-uniform trip
+// counts) of TX words each, read in one batch per row (taps past the
+// sample's range have weight 0; the row window stays inside the sub-grid
+// and its zeroed slack). Weight ((wz*wy)*wx) as the reference forms it.
+template <int DIM, int NX, int CB, int TX>
+__device__ inline void interp_rows(const float2* sub, int P, int Pmax, int bx, int by, int bz, int ny,
+                                   int nz, const float* w, int c, float* accx, float* accy) {
+  float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+  for (int tz = 0; tz < (DIM == 3 ? NX : 1); tz++) {
+    if (tz >= nz) break;
+    const int iz = DIM == 3 ? min(bz + tz, Pmax) : 0;
+    const float wz = DIM == 3 ? w[1 + 2 * NX + tz] : 1.0f;
+#pragma unroll
+    for (int ty = 0; ty < NX; ty++) {
+      if (ty >= ny) break;
+      const int iy = min(by + ty, Pmax);
+      const float wzy = DIM == 3 ? wz * w[1 + NX + ty] : w[1 + NX + ty];
+      const float2* row = sub + ((iz * P + iy) * P + bx) * CB + c;
+      float2 v[TX];
+#pragma unroll
+      for (int tx = 0; tx < TX; tx++) v[tx] = row[tx * CB];
+#pragma unroll
+      for (int tx = 0; tx < TX; tx++) {
+        const float val = wzy * w[1 + tx];
+        ax += v[tx].x * val;
+        ay += v[tx].y * val;
+      }
+    }
+  }
+  *accx = ax;
+  *accy = ay;
+}
+
 template <int DIM, int NX, int CB, int NT = kThreads>
 __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t ci, int coil_block,
                                    int R) {
@@ -1254,31 +1301,12 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
       const int ny = wave_trip(packed, 1, g.kw);
       const int nz = DIM == 3 ? wave_trip(packed, 2, g.kw) : 1;
       float accx = 0.0f, accy = 0.0f;
-#pragma unroll
-      for (int tz = 0; tz < (DIM == 3 ? NX : 1); tz++) {
-        if (tz >= nz) break;
-        const int iz = DIM == 3 ? min(bz + tz, Pmax) : 0;
-        const float wz = DIM == 3 ? w[1 + 2 * NX + tz] : 1.0f;
-#pragma unroll
-        for (int ty = 0; ty < NX; ty++) {
-          if (ty >= ny) break;
-          const int iy = min(by + ty, Pmax);
-          const float wzy = DIM == 3 ? wz * w[1 + NX + ty] : w[1 + NX + ty];
-          const float2* row = sub + ((iz * P + iy) * P + bx) * CB + c;
-          // all NX words of the row in one batch of reads (taps past the
-          // sample's range have weight 0; the row window stays inside the
-          // sub-grid and its zeroed slack)
-          float2 v[NX];
-#pragma unroll
-          for (int tx = 0; tx < NX; tx++) v[tx] = row[tx * CB];
-#pragma unroll
-          for (int tx = 0; tx < NX; tx++) {
-            const float val = wzy * w[1 + tx];
-            accx += v[tx].x * val;
-            accy += v[tx].y * val;
-          }
-        }
-      }
+      // x taps: NX, or 3 when NX is 4 and no lane of the wave has a fourth
+      // (wave-uniform; one branch per sample, the rows unrolled)
+      if (NX == 4 && wave_trip(packed, 0, g.kw) <= 3)
+        interp_rows<DIM, NX, CB, (NX == 4 ? 3 : NX)>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
+      else
+        interp_rows<DIM, NX, CB, NX>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
       const uint32_t s = ch.begin + r0 + ls;
       float rx = accx * a.scale, ry = accy * a.scale;
       if (a.dens) {
@@ -1695,6 +1723,17 @@ int fold_threads(const Geom& g, const Cover& cover) {
   return threads;
 }
 
+template <int DIM, bool GEN>
+static void fold_launch(int fcb, dim3 grid, int threads, const Geom& g, const SpreadArgs& a,
+                        hipStream_t s) {
+  switch (fcb) {
+    case 8: hipLaunchKernelGGL((fold_kernel<DIM, 8, GEN>), grid, dim3(threads), 0, s, g, a); break;
+    case 4: hipLaunchKernelGGL((fold_kernel<DIM, 4, GEN>), grid, dim3(threads), 0, s, g, a); break;
+    case 2: hipLaunchKernelGGL((fold_kernel<DIM, 2, GEN>), grid, dim3(threads), 0, s, g, a); break;
+    default: hipLaunchKernelGGL((fold_kernel<DIM, 1, GEN>), grid, dim3(threads), 0, s, g, a); break;
+  }
+}
+
 hipError_t launch_spread(const Geom& g, const SpreadArgs& args, int chunk, hipStream_t s) {
   if (args.n_coils == 0) return hipSuccess;
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
@@ -1726,21 +1765,11 @@ hipError_t launch_spread(const Geom& g, const SpreadArgs& args, int chunk, hipSt
   if (threads == 0) return hipErrorInvalidValue;
   const int fcb = a.n_coils >= 8 ? 8 : a.n_coils >= 4 ? 4 : a.n_coils >= 2 ? 2 : 1;
   const dim3 grid(S, (a.n_coils + fcb - 1) / fcb);
-  if (g.dim == 3) {
-    switch (fcb) {
-      case 8: hipLaunchKernelGGL((fold_kernel<3, 8>), grid, dim3(threads), 0, s, g, a); break;
-      case 4: hipLaunchKernelGGL((fold_kernel<3, 4>), grid, dim3(threads), 0, s, g, a); break;
-      case 2: hipLaunchKernelGGL((fold_kernel<3, 2>), grid, dim3(threads), 0, s, g, a); break;
-      default: hipLaunchKernelGGL((fold_kernel<3, 1>), grid, dim3(threads), 0, s, g, a); break;
-    }
-  } else {
-    switch (fcb) {
-      case 8: hipLaunchKernelGGL((fold_kernel<2, 8>), grid, dim3(threads), 0, s, g, a); break;
-      case 4: hipLaunchKernelGGL((fold_kernel<2, 4>), grid, dim3(threads), 0, s, g, a); break;
-      case 2: hipLaunchKernelGGL((fold_kernel<2, 2>), grid, dim3(threads), 0, s, g, a); break;
-      default: hipLaunchKernelGGL((fold_kernel<2, 1>), grid, dim3(threads), 0, s, g, a); break;
-    }
-  }
+  const bool gen = fold_generic(a.cover);
+  if (g.dim == 3)
+    gen ? fold_launch<3, true>(fcb, grid, threads, g, a, s) : fold_launch<3, false>(fcb, grid, threads, g, a, s);
+  else
+    gen ? fold_launch<2, true>(fcb, grid, threads, g, a, s) : fold_launch<2, false>(fcb, grid, threads, g, a, s);
   return hipGetLastError();
 }
 
@@ -1800,13 +1829,13 @@ __global__ void __launch_bounds__(kBatchThreads) interp_batch_kernel(const Batch
   interp_body<DIM, NX, kBatchCB, kBatchThreads>(op.g, op.i, it.y, (int)it.z, op.R_interp);
 }
 
-template <int DIM>
+template <int DIM, bool GEN>
 __global__ void __launch_bounds__(kFoldThreads) fold_batch_kernel(const BatchOp* __restrict__ ops,
                                                                   BatchList l) {
   uint4 it;
   if (!batch_item(l, blockIdx.x, &it)) return;
   const BatchOp& op = ops[it.x];
-  fold_body<DIM, kBatchCB>(op.g, op.s, it.y, (int)it.z);
+  fold_body<DIM, kBatchCB, GEN>(op.g, op.s, it.y, (int)it.z);
 }
 
 size_t lds_budget() { return lds_limit(); }
@@ -1906,9 +1935,12 @@ hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchL
 }
 
 hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, int threads,
-                             hipStream_t s) {
-  return dim == 3 ? launch_items(fold_batch_kernel<3>, ops, l, threads, 0, s)
-                  : launch_items(fold_batch_kernel<2>, ops, l, threads, 0, s);
+                             bool generic, hipStream_t s) {
+  if (generic)
+    return dim == 3 ? launch_items(fold_batch_kernel<3, true>, ops, l, threads, 0, s)
+                    : launch_items(fold_batch_kernel<2, true>, ops, l, threads, 0, s);
+  return dim == 3 ? launch_items(fold_batch_kernel<3, false>, ops, l, threads, 0, s)
+                  : launch_items(fold_batch_kernel<2, false>, ops, l, threads, 0, s);
 }
 
 
