@@ -327,6 +327,10 @@ void partition_items(const uint32_t* costs, uint32_t n, BatchList* l);
 constexpr int kBatchCB = 8;
 constexpr int kBatchLaneCB = 4;  // coil block of the batched lane (kernel width 3) kernels
 constexpr int kBatchThreads = 512;
+// Batched lane spread: chunks of at most 2 * kBatchSmallThreads samples run
+// in workgroups of kBatchSmallThreads (sparse 2-D plans are mostly such
+// chunks, and a 512-thread workgroup would leave most of its waves idle).
+constexpr int kBatchSmallThreads = 128;
 constexpr int kBatchPerCU = 4;
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
@@ -337,13 +341,18 @@ hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, in
 bool fold_generic(const Cover& cover);
 hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
+// lane spread in kBatchSmallThreads workgroups, for chunks of at most
+// 2 * kBatchSmallThreads samples
+hipError_t launch_spread_batch_small(int dim, const BatchOp* ops, const BatchList& l, size_t lds,
+                                     hipStream_t s);
 size_t spread_lds_bytes(const Geom& g, int cb, int R);
 // Batched lane kernels (launch_*_batch with nx == 3): usable for plan g;
 // their dynamic LDS for a coil block of cb.
 bool lane_batch_ok(const Geom& g);
 hipError_t launch_reduce_batch(const BatchOp* ops, const BatchList& l, uint32_t max_kstride,
                                hipStream_t s);
-size_t lane_lds_bytes(const Geom& g, int cb);
+// lut: with the kernel table copied to LDS (the small-workgroup spread reads it from global memory)
+size_t lane_lds_bytes(const Geom& g, int cb, bool lut = true);
 size_t interp_lds_bytes(const Geom& g, int cb, int R);
 int spread_round(const Geom& g, int cb, int chunk, size_t limit);
 int interp_round(const Geom& g, int cb, int chunk, size_t limit);

@@ -511,11 +511,12 @@ __global__ void __launch_bounds__(kThreads) spread_kernel(Geom g, SpreadArgs a, 
 struct LaneLayout {
   int kpad, kpw, sub_n;
   size_t kp_off, sub_off, bytes;
-  __host__ __device__ LaneLayout(const Geom& g, int kw, int cb) {
+  // lut: the kernel table is copied to LDS (else read from global memory)
+  __host__ __device__ LaneLayout(const Geom& g, int kw, int cb, bool lut = true) {
     const int P = g.P;
     const int PD = g.dim == 3 ? P * P * P : P * P;
     const int slack = g.dim == 3 ? 2 * P * P + 2 * P + 2 : 2 * P + 2;
-    kpad = (g.K + 3) & ~3;
+    kpad = lut ? (g.K + 3) & ~3 : 0;
     kpw = (3 * (P + kw + 1) + 3) & ~3;
     sub_n = (PD + slack) * cb;
     kp_off = (size_t)kpad * 4;
@@ -588,7 +589,10 @@ __device__ inline uint32_t quad_pass_rt(uint32_t v, int ps) {
   }
 }
 
-template <int DIM, int KW, int CB, int NT = kThreads>
+// LUTG: kernel table read from global memory (cached) instead of an LDS
+// copy, for small workgroups whose few samples make fewer lookups than the
+// copy has words.
+template <int DIM, int KW, int CB, int NT = kThreads, bool LUTG = false>
 __device__ inline void spread_lane_body(const Geom& g, const SpreadArgs& a, uint32_t ci,
                                         int coil_block) {
   constexpr int NW = NT / 64;
@@ -603,7 +607,7 @@ __device__ inline void spread_lane_body(const Geom& g, const SpreadArgs& a, uint
   __shared__ double scale_s[CB];
   __shared__ double inv_s[CB];
   char* smem = reinterpret_cast<char*>(smem4);
-  const LaneLayout L(g, KW, CB);
+  const LaneLayout L(g, KW, CB, !LUTG);
   float* lut = reinterpret_cast<float*>(smem);
   float* kp = reinterpret_cast<float*>(smem + L.kp_off);
   unsigned long long* sub = reinterpret_cast<unsigned long long*>(smem + L.sub_off);
@@ -667,7 +671,8 @@ __device__ inline void spread_lane_body(const Geom& g, const SpreadArgs& a, uint
 
   int c3[3];
   load_center<DIM>(g, ch.sector, c3);
-  copy_to_lds<NT, 4>(lut, a.lut, g.K);
+  if (!LUTG) copy_to_lds<NT, 4>(lut, a.lut, g.K);
+  const float* wlut = LUTG ? a.lut : lut;
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
   {
     ulonglong2* sub2 = reinterpret_cast<ulonglong2*>(sub);
@@ -723,7 +728,7 @@ __device__ inline void spread_lane_body(const Geom& g, const SpreadArgs& a, uint
 #pragma unroll
         for (int t = 0; t < NX; t++) {
           float wt = 0.0f;
-          if (lo + t <= hi && axis_weight_kp(g, lut, d, kp[d * KPS + lo + t], p[r][d], &wt))
+          if (lo + t <= hi && axis_weight_kp(g, wlut, d, kp[d * KPS + lo + t], p[r][d], &wt))
             w[d][t] = wt;
         }
       }
@@ -1896,11 +1901,26 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
   interp_lane_body<DIM, 3, kBatchLaneCB, kBatchThreads>(op.g, op.i, it.y, (int)it.z);
 }
 
+template <int DIM>
+__global__ void __launch_bounds__(kBatchSmallThreads) __attribute__((amdgpu_waves_per_eu(6)))
+    spread_batch_lane_small_kernel(const BatchOp* __restrict__ ops, BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  spread_lane_body<DIM, 3, kBatchLaneCB, kBatchSmallThreads, true>(op.g, op.s, it.y, (int)it.z);
+}
+
+hipError_t launch_spread_batch_small(int dim, const BatchOp* ops, const BatchList& l, size_t lds,
+                                     hipStream_t s) {
+  return dim == 3 ? launch_items(spread_batch_lane_small_kernel<3>, ops, l, kBatchSmallThreads, lds, s)
+                  : launch_items(spread_batch_lane_small_kernel<2>, ops, l, kBatchSmallThreads, lds, s);
+}
+
 bool lane_batch_ok(const Geom& g) {
   return lane_spread_ok(g, 2 * kBatchThreads) && lane_interp_ok(g, 2 * kBatchThreads);
 }
 
-size_t lane_lds_bytes(const Geom& g, int cb) { return LaneLayout(g, 3, cb).bytes; }
+size_t lane_lds_bytes(const Geom& g, int cb, bool lut) { return LaneLayout(g, 3, cb, lut).bytes; }
 
 hipError_t launch_spread_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s) {
