@@ -1366,10 +1366,10 @@ struct gnufft_batch_s {
   BatchOp* ops_d = nullptr;
   uint4* items_d = nullptr;
   BatchList spread{}, interp{}, fold{}, reduce{}, post{}, pre{};
-  // lane spread: chunks of at most 2 * kBatchSmallThreads samples, run in
-  // small workgroups (the interp runs every chunk in the full-size kernel)
-  BatchList spread_small{};
-  size_t spread_lds = 0, interp_lds = 0, spread_small_lds = 0;
+  // lane kernels: chunks of at most 2 * kBatchSmallThreads samples, run in
+  // small workgroups (spread and interp share the lists)
+  BatchList spread_small{}, interp_small{};
+  size_t spread_lds = 0, interp_lds = 0, spread_small_lds = 0, interp_small_lds = 0;
   int fold_threads = 64;
   bool fold_generic = false;  // some plan needs the fold's generic body
   uint32_t max_kstride = 0;
@@ -1550,10 +1550,9 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     return fail(GNUFFT_ERR_OUT_OF_MEMORY, "batch_create: grid / scratch allocation");
   }
   // descriptors and item lists
-  // sp: every (chunk, coil block), the interp's list; sl / ss: the spread's
-  // lists of large and small chunks
-  std::vector<uint4> sp, sl, ss, rd, fo, po, pr;
-  std::vector<uint32_t> sp_cost, sl_cost, ss_cost, fo_cost;
+  // (chunk, coil block) items of large (sp) and small (ss) chunks
+  std::vector<uint4> sp, ss, rd, fo, po, pr;
+  std::vector<uint32_t> sp_cost, ss_cost, fo_cost;
   for (uint32_t k = 0; k < n_plans && !rc; k++) {
     const gnufft_plan p = b->plans[k];
     const uint32_t C = b->coils[k];
@@ -1590,6 +1589,7 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
     if (lane) {
       b->spread_lds = std::max(b->spread_lds, lane_lds_bytes(p->g, kBatchLaneCB));
       b->spread_small_lds = std::max(b->spread_small_lds, lane_lds_bytes(p->g, kBatchLaneCB, false));
+      b->interp_small_lds = b->spread_small_lds;
       b->interp_lds = std::max(b->interp_lds, lane_lds_bytes(p->g, kBatchLaneCB));
     } else {
       b->spread_lds = std::max(b->spread_lds, spread_lds_bytes(p->g, kBatchCB, op.R_spread));
@@ -1625,15 +1625,8 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
       const uint32_t n = ch[ci].end - ch[ci].begin;
       const bool small = lane && n <= 2u * (uint32_t)kBatchSmallThreads;
       for (uint32_t cb = 0; cb < cbs; cb++) {
-        sp.push_back(make_uint4(k, ci, cb, 0));
-        sp_cost.push_back(n + 300u);
-        if (!small) {
-          sl.push_back(make_uint4(k, ci, cb, 0));
-          sl_cost.push_back(n + 300u);
-        } else {
-          ss.push_back(make_uint4(k, ci, cb, 0));
-          ss_cost.push_back(n + 100u);
-        }
+        (small ? ss : sp).push_back(make_uint4(k, ci, cb, 0));
+        (small ? ss_cost : sp_cost).push_back(n + (small ? 100u : 300u));
       }
     }
     for (uint32_t i = 0; i < p->n_chunks;) {
@@ -1656,7 +1649,6 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
   if (!rc) {
     std::vector<uint4> all;
     all.insert(all.end(), sp.begin(), sp.end());
-    all.insert(all.end(), sl.begin(), sl.end());
     all.insert(all.end(), ss.begin(), ss.end());
     all.insert(all.end(), rd.begin(), rd.end());
     all.insert(all.end(), fo.begin(), fo.end());
@@ -1681,9 +1673,10 @@ int gnufft_batch_create(gnufft_batch* batch, const gnufft_plan* plans, uint32_t 
       }
       at += v.size();
     };
-    list(b->interp, sp, &sp_cost);
-    list(b->spread, sl, &sl_cost);
+    list(b->spread, sp, &sp_cost);
+    b->interp = b->spread;
     list(b->spread_small, ss, &ss_cost);
+    b->interp_small = b->spread_small;
     list(b->reduce, rd, nullptr);
     list(b->fold, fo, &fo_cost);
     list(b->post, po, nullptr);
@@ -1819,6 +1812,7 @@ int gnufft_batch_forward(gnufft_batch b, const void* const* img, void* const* ou
     }
   }
   HIP_TRY(launch_interp_batch(b->dim, b->nx, b->ops_d, b->interp, b->interp_lds, s));
+  HIP_TRY(launch_interp_batch_small(b->dim, b->ops_d, b->interp_small, b->interp_small_lds, s));
   return batch_mark_done(b, s);
 }
 

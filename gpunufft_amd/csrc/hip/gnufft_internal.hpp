@@ -327,7 +327,7 @@ void partition_items(const uint32_t* costs, uint32_t n, BatchList* l);
 constexpr int kBatchCB = 8;
 constexpr int kBatchLaneCB = 4;  // coil block of the batched lane (kernel width 3) kernels
 constexpr int kBatchThreads = 512;
-// Batched lane spread: chunks of at most 2 * kBatchSmallThreads samples run
+// Batched lane kernels: chunks of at most 2 * kBatchSmallThreads samples run
 // in workgroups of kBatchSmallThreads (sparse 2-D plans are mostly such
 // chunks, and a 512-thread workgroup would leave most of its waves idle).
 constexpr int kBatchSmallThreads = 128;
@@ -341,9 +341,11 @@ hipError_t launch_fold_batch(int dim, const BatchOp* ops, const BatchList& l, in
 bool fold_generic(const Cover& cover);
 hipError_t launch_interp_batch(int dim, int nx, const BatchOp* ops, const BatchList& l,
                                size_t lds, hipStream_t s);
-// lane spread in kBatchSmallThreads workgroups, for chunks of at most
+// lane kernels in kBatchSmallThreads workgroups, for chunks of at most
 // 2 * kBatchSmallThreads samples
 hipError_t launch_spread_batch_small(int dim, const BatchOp* ops, const BatchList& l, size_t lds,
+                                     hipStream_t s);
+hipError_t launch_interp_batch_small(int dim, const BatchOp* ops, const BatchList& l, size_t lds,
                                      hipStream_t s);
 size_t spread_lds_bytes(const Geom& g, int cb, int R);
 // Batched lane kernels (launch_*_batch with nx == 3): usable for plan g;

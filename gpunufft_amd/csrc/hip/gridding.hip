@@ -1368,7 +1368,8 @@ __device__ inline void lane_gather(const float2* sub, int P, const int* b, const
   *ay = sy;
 }
 
-template <int DIM, int KW, int CB, int NT = kThreads>
+// LUTG as in spread_lane_body.
+template <int DIM, int KW, int CB, int NT = kThreads, bool LUTG = false>
 __device__ inline void interp_lane_body(const Geom& g, const InterpArgs& a, uint32_t ci,
                                         int coil_block) {
   constexpr int NX = KW + 1;
@@ -1378,7 +1379,7 @@ __device__ inline void interp_lane_body(const Geom& g, const InterpArgs& a, uint
 
   extern __shared__ float4 smem4[];
   char* smem = reinterpret_cast<char*>(smem4);
-  const LaneLayout L(g, KW, CB);
+  const LaneLayout L(g, KW, CB, !LUTG);
   float* lut = reinterpret_cast<float*>(smem);
   float* kp = reinterpret_cast<float*>(smem + L.kp_off);
   float2* sub = reinterpret_cast<float2*>(smem + L.sub_off);
@@ -1407,7 +1408,8 @@ __device__ inline void interp_lane_body(const Geom& g, const InterpArgs& a, uint
     if (live[r]) load_coord<DIM>(a.crds, g.M, ch.begin + (uint32_t)j, p[r]);
   }
 
-  copy_to_lds<NT, 4>(lut, a.lut, g.K);
+  if (!LUTG) copy_to_lds<NT, 4>(lut, a.lut, g.K);
+  const float* wlut = LUTG ? a.lut : lut;
   fill_kpos<DIM, NT>(g, c3, kp, KPS);
   // Stage the padded sector as [point][coil] (periodic wrap and the FORWARD
   // fftshift folded into the grid index), zeros in the slack past it (its
@@ -1468,7 +1470,7 @@ __device__ inline void interp_lane_body(const Geom& g, const InterpArgs& a, uint
 #pragma unroll
         for (int t = 0; t < NX; t++) {
           float wt = 0.0f;
-          if (lo + t <= hi && axis_weight_kp(g, lut, d, kp[d * KPS + lo + t], p[r][d], &wt))
+          if (lo + t <= hi && axis_weight_kp(g, wlut, d, kp[d * KPS + lo + t], p[r][d], &wt))
             w[d][t] = wt;
         }
       }
@@ -1908,6 +1910,21 @@ __global__ void __launch_bounds__(kBatchSmallThreads) __attribute__((amdgpu_wave
   if (!batch_item(l, blockIdx.x, &it)) return;
   const BatchOp& op = ops[it.x];
   spread_lane_body<DIM, 3, kBatchLaneCB, kBatchSmallThreads, true>(op.g, op.s, it.y, (int)it.z);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBatchSmallThreads) __attribute__((amdgpu_waves_per_eu(6)))
+    interp_batch_lane_small_kernel(const BatchOp* __restrict__ ops, BatchList l) {
+  uint4 it;
+  if (!batch_item(l, blockIdx.x, &it)) return;
+  const BatchOp& op = ops[it.x];
+  interp_lane_body<DIM, 3, kBatchLaneCB, kBatchSmallThreads, true>(op.g, op.i, it.y, (int)it.z);
+}
+
+hipError_t launch_interp_batch_small(int dim, const BatchOp* ops, const BatchList& l, size_t lds,
+                                     hipStream_t s) {
+  return dim == 3 ? launch_items(interp_batch_lane_small_kernel<3>, ops, l, kBatchSmallThreads, lds, s)
+                  : launch_items(interp_batch_lane_small_kernel<2>, ops, l, kBatchSmallThreads, lds, s);
 }
 
 hipError_t launch_spread_batch_small(int dim, const BatchOp* ops, const BatchList& l, size_t lds,
