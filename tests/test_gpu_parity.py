@@ -230,6 +230,42 @@ def test_operator_all_stages(oracle, torch_dev, capi, case, use_dens, use_sens, 
     assert oracle.rel_l2(host(out), ref) < TOL, "forward"
 
 
+# 3-D grids of 96 points along z (compile-time FFT kernels for the z
+# pass): adjoint FFT and DEAPODIZATION stops with density and coil maps,
+# partial and full coil blocks, sector widths 8 and 16, an anisotropic grid
+FOLD_FFT_CASES = [
+    ("koosh3d_64_sw8", lambda: W.koosh_3d(900, 64), (64, 64, 64), 1.5, 3, 8),
+    ("koosh3d_64_sw16", lambda: W.koosh_3d(900, 64), (64, 64, 64), 1.5, 3, 16),
+    ("rand3d_aniso_z96", lambda: np.random.default_rng(11).uniform(-0.5, 0.5, (3, 30000)).astype(np.float32),
+     (40, 48, 64), 1.5, 3, 8),
+]
+
+
+@pytest.mark.parametrize("C", [3, 8])
+@pytest.mark.parametrize("case", FOLD_FFT_CASES, ids=lambda c: c[0])
+def test_adjoint_fixed_z_pass(oracle, torch_dev, capi, case, C):
+    name, tf, img, osf, kw, sw = case
+    print("case", name, C, flush=True)
+    traj = tf()
+    M = traj.shape[1]
+    rng = np.random.default_rng(C)
+    dens = W.ramp_density(traj)
+    sens = W.gaussian_sens(img, C, seed=4)
+    kd = W.complex_normal(rng, (C, M))
+    k_dev = dev(torch_dev, kd)
+    for use_sens in (False, True):
+        pl, op = _op_pair(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, 0, dens,
+                          sens if use_sens else None)
+        ishape = oracle.img_shape(op.gi)
+        for stop in (oracle.FFT, oracle.DEAPODIZATION):
+            ref = op.adjoint(kd, stop=stop)
+            per_coil = not (use_sens and stop == oracle.DEAPODIZATION)
+            out = torch_dev.empty(((C,) if per_coil else (1,)) + ishape, dtype=torch_dev.complex64,
+                                  device="cuda")
+            pl.adjoint(k_dev, out, C, stop)
+            assert oracle.rel_l2(host(out), ref) < TOL, (name, C, use_sens, stop)
+
+
 @pytest.mark.parametrize("case", [CASES[1], CASES[4], CASES[7]], ids=lambda c: c[0])
 def test_load_precomputed_std_order(oracle, torch_dev, capi, case):
     """Precomputed tables in the reference's std::sort order (as a
