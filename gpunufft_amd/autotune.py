@@ -18,6 +18,9 @@ repeat count.
 
 Workgroup size and coil blocking follow from the width: the kernels size
 their LDS rounds and coil blocks to the padded sector of the chosen width.
+For one plan, best_workgroups_per_cu then times the kernel-width-3 gridding
+kernels at each workgroups-per-CU target (the coil block they pick: 4, 2
+or 1 coils of the padded sector in LDS) and keeps the fastest on the plan.
 """
 
 from __future__ import annotations
@@ -76,4 +79,40 @@ def best_sector_width(img_dims, kernel_width, osf, traj, dens, n_coils, widths=(
     if best is None:
         raise capi.GnufftError(capi.OK + 5, "no sector width of %s fits this operator" % (widths,))
     _CACHE[key] = best
+    return best
+
+
+def best_workgroups_per_cu(plan, n_coils, targets=(2, 3, 4), reps=3):
+    """plan: a capi.Plan with its trajectory set. Times spread + interp of
+    n_coils coils (sorted-order entry points, device-resident zeros) at each
+    workgroups-per-CU target, sets the fastest on the plan and returns it.
+    The gridded values do not depend on the target."""
+    import torch
+
+    info = plan.info()
+    M = int(info.n_samples)
+    dims = int(info.dim)
+    gshape = (n_coils,) + tuple(int(x) for x in info.grid[:dims][::-1])
+    data = torch.zeros((n_coils, M), dtype=torch.complex64, device="cuda")
+    grid = torch.empty(gshape, dtype=torch.complex64, device="cuda")
+    out = torch.empty((n_coils, M), dtype=torch.complex64, device="cuda")
+    best, best_t = None, None
+    for n in targets:
+        plan.set_workgroups_per_cu(n)
+        plan.spread_sorted(data, grid, n_coils)
+        plan.interp_sorted(grid, out, n_coils)
+        ts = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            plan.spread_sorted(data, grid, n_coils)
+            plan.interp_sorted(grid, out, n_coils)
+            b.record()
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        t = min(ts)
+        if best_t is None or t < best_t:
+            best, best_t = n, t
+    plan.set_workgroups_per_cu(best)
     return best

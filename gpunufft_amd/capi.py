@@ -32,7 +32,8 @@ EXPORTS = [
     "gnufft_spread_sorted", "gnufft_interp_sorted", "gnufft_plan_set_timing",
     "gnufft_plan_get_timing", "gnufft_batch_create", "gnufft_batch_destroy",
     "gnufft_batch_adjoint", "gnufft_batch_forward", "gnufft_cg_create", "gnufft_cg_destroy",
-    "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft", "gnufft_check_precomputed",
+    "gnufft_cg_solve", "gnufft_plan_set_fft", "gnufft_plan_get_fft",
+    "gnufft_plan_set_workgroups_per_cu", "gnufft_check_precomputed",
     "gnufft_adjoint_noalloc",
     "gnufft_noalloc_capacity",
 ]
@@ -122,6 +123,7 @@ def load(path=LIB_PATH):
     L.gnufft_interp_sorted.argtypes = [vp, vp, vp, u32, vp]
     L.gnufft_plan_set_fft.argtypes = [vp, i32]
     L.gnufft_plan_get_fft.argtypes = [vp, ctypes.POINTER(i32)]
+    L.gnufft_plan_set_workgroups_per_cu.argtypes = [vp, i32]
     L.gnufft_plan_set_timing.argtypes = [vp, i32]
     L.gnufft_plan_get_timing.argtypes = [vp, ctypes.POINTER(Timing), i32]
     vpp = ctypes.POINTER(vp)
@@ -282,6 +284,11 @@ class Plan:
         check(load().gnufft_spread_sorted(self._h, _ptr_loc(data_sorted)[0], _ptr_loc(grid)[0],
                                           n_coils, _stream_ptr(stream)))
         return grid
+
+    def set_workgroups_per_cu(self, n):
+        """Workgroups per CU the kernel-width-3 gridding kernels size their
+        coil block for (0: default)."""
+        check(load().gnufft_plan_set_workgroups_per_cu(self._h, int(n)))
 
     def set_fft(self, kind):
         """FFT_FUSED (pruned per-axis passes with the pre / post stages

@@ -247,6 +247,7 @@ struct gnufft_plan_s {
   // workgroups per CU), at most `chunk`
   uint32_t round_target = 1024;
   WorkList work{};
+  int lane_per_cu = 0;  // gnufft_plan_set_workgroups_per_cu (0: default)
   uint32_t* sector_chunks_d = nullptr;  // [S+1] first chunk of each sector
   uint2* reduce_d = nullptr;            // (first chunk, count) of multi-chunk sectors
   uint32_t n_reduce = 0;
@@ -654,6 +655,7 @@ SpreadArgs spread_args(gnufft_plan p, const float2* data, float2* grid, uint32_t
   a.cover = p->cover;
   a.grid = grid;
   a.n_coils = (int)n_coils;
+  a.lane_per_cu = p->lane_per_cu;
   return a;
 }
 
@@ -843,6 +845,13 @@ int gnufft_plan_set_fft(gnufft_plan p, int kind) {
   return GNUFFT_OK;
 }
 
+
+int gnufft_plan_set_workgroups_per_cu(gnufft_plan p, int n) {
+  if (!p) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_workgroups_per_cu: null plan");
+  if (n < 0 || n > 8) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "set_workgroups_per_cu: n must be 0..8");
+  p->lane_per_cu = n;
+  return GNUFFT_OK;
+}
 
 int gnufft_plan_get_fft(gnufft_plan p, int* kind) {
   if (!p || !kind) return fail(GNUFFT_ERR_INVALID_ARGUMENT, "get_fft: null argument");
@@ -1212,6 +1221,7 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   a.grid = (const float2*)grid;
   a.crds = p->traj_sorted_d;
   a.work = p->work;
+  a.lane_per_cu = p->lane_per_cu;
   a.lut = p->lut_d;
   a.out = (float2*)data_sorted;
   a.coil_stride = p->M;
@@ -1304,6 +1314,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.grid = p->img_tmp_d;
     a.crds = p->traj_sorted_d;
     a.work = p->work;
+    a.lane_per_cu = p->lane_per_cu;
     a.lut = p->lut_d;
     a.out = (float2*)out;
     a.coil_stride = p->M;
@@ -1338,6 +1349,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.grid = p->grid_d;
     a.crds = p->traj_sorted_d;
     a.work = p->work;
+    a.lane_per_cu = p->lane_per_cu;
     a.lut = p->lut_d;
     a.out = (float2*)out + (size_t)c0 * p->M;
     a.coil_stride = p->M;

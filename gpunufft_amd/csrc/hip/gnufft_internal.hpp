@@ -241,6 +241,7 @@ struct SpreadArgs {
   float2* grid;            // [coil][Ng], every point written (no prior zeroing)
   int n_coils;
   bool fold_shift;         // store at INVERSE-fftshifted positions
+  int lane_per_cu;         // lane kernels: workgroups per CU their coil block is sized for (0: default)
 };
 hipError_t launch_spread(const Geom& g, const SpreadArgs& a, int chunk, hipStream_t s);
 
@@ -256,6 +257,7 @@ struct InterpArgs {
   float scale;             // 1 (kernel only) or 1/sqrt(Ng)
   int n_coils;
   bool fold_shift;         // read grid at FORWARD-fftshifted positions
+  int lane_per_cu;         // as in SpreadArgs
 };
 hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s);
 

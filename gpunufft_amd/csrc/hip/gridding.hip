@@ -1667,7 +1667,8 @@ static int lane_per_cu() {
 template <int DIM>
 static hipError_t spread_lane(const Geom& g, SpreadArgs a, hipStream_t s, int* cb_out) {
   int cb = 0;
-  for (int per = lane_per_cu(); per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
+  const int first = a.lane_per_cu > 0 ? a.lane_per_cu : lane_per_cu();
+  for (int per = first; per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
   a.sub_cb = cb;
   *cb_out = cb;
   switch (cb) {
@@ -1795,7 +1796,8 @@ static hipError_t interp_lane_launch(const Geom& g, const InterpArgs& a, hipStre
 template <int DIM>
 static hipError_t interp_lane(const Geom& g, const InterpArgs& a, hipStream_t s) {
   int cb = 0;
-  for (int per = lane_per_cu(); per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
+  const int first = a.lane_per_cu > 0 ? a.lane_per_cu : lane_per_cu();
+  for (int per = first; per >= 1 && cb == 0; per--) cb = lane_cb(g, a.n_coils, per);
   switch (cb) {
     case 4: return interp_lane_launch<DIM, 4>(g, a, s);
     case 2: return interp_lane_launch<DIM, 2>(g, a, s);

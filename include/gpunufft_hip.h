@@ -124,6 +124,16 @@ enum { GNUFFT_FFT_FUSED = 0, GNUFFT_FFT_ROCFFT = 1 };
 int gnufft_plan_set_fft(gnufft_plan plan, int kind);
 int gnufft_plan_get_fft(gnufft_plan plan, int* kind);
 
+/* Launch shape of a plan's kernel-width-3 gridding kernels. The reference
+ * launches fixed 256-thread blocks, one per sector (THREAD_BLOCK_SIZE,
+ * CUDA/inc/cuda_utils.cuh:75-77; getOptimalGridDim, cuda_utils.hpp:155).
+ * Here the spread and interp workgroups size their coil block (4, 2 or 1
+ * coils of a padded sector in LDS) for n workgroups per CU, falling back
+ * to fewer when the sector does not fit: n = 0 restores the default (3, or
+ * the GNUFFT_LANE_PER_CU environment variable), 1..8 sets it for this
+ * plan. Results do not depend on it; gpunufft_amd.autotune times it. */
+int gnufft_plan_set_workgroups_per_cu(gnufft_plan plan, int n);
+
 /* Sector precompute on the device: assignSectors + sort + sectorDataCount +
  * sorted trajectory/density (CUDA/src/gpuNUFFT_operator_factory.cpp:128-205,
  * :450-553; CUDA/src/gpu/precomp_kernels.cu:8-177). traj/dens live at

@@ -230,6 +230,48 @@ def test_operator_all_stages(oracle, torch_dev, capi, case, use_dens, use_sens, 
     assert oracle.rel_l2(host(out), ref) < TOL, "forward"
 
 
+@pytest.mark.parametrize("case", [CASES[0], CASES[4], CASES[7]], ids=lambda c: c[0])
+def test_workgroups_per_cu_same_result(oracle, torch_dev, capi, case):
+    """The workgroups-per-CU target changes the coil block of the
+    kernel-width-3 kernels (4, 2 or 1 coils), not the result: the
+    sub-grid sums are exact integers and the interp sums run in a fixed
+    order; only the float L1 behind each chunk's fixed-point scale is summed
+    over other lanes, so outputs agree with the default's to rel-L2 1e-6.
+    Out-of-range targets are refused, and the autotuner keeps one of its
+    candidates."""
+    from gpunufft_amd import autotune
+
+    name, tf, img, osf, kw, sw = case
+    traj = tf()
+    M = traj.shape[1]
+    C = 5
+    rng = np.random.default_rng(3)
+    kd = dev(torch_dev, W.complex_normal(rng, (C, M)))
+    pl = capi.Plan(img, kw, sw, osf, 0)
+    pl.set_trajectory(dev(torch_dev, traj), dev(torch_dev, W.ramp_density(traj)))
+    info = pl.info()
+    ishape = tuple(int(x) for x in info.img[:len(img)][::-1])
+    x = dev(torch_dev, W.complex_normal(rng, (C,) + ishape))
+
+    def run():
+        img_out = torch_dev.empty((C,) + ishape, dtype=torch_dev.complex64, device="cuda")
+        k_out = torch_dev.empty((C, M), dtype=torch_dev.complex64, device="cuda")
+        pl.adjoint(kd, img_out, C, oracle.DEAPODIZATION)
+        pl.forward(x, k_out, C)
+        return host(img_out), host(k_out)
+
+    ref = run()
+    for n in (1, 2, 3, 4):
+        pl.set_workgroups_per_cu(n)
+        got = run()
+        assert oracle.rel_l2(got[0], ref[0]) < 1e-6 and oracle.rel_l2(got[1], ref[1]) < 1e-6, (name, n)
+    for bad in (-1, 9):
+        with pytest.raises(capi.GnufftError):
+            pl.set_workgroups_per_cu(bad)
+    assert autotune.best_workgroups_per_cu(pl, C) in (2, 3, 4)
+    assert oracle.rel_l2(run()[0], ref[0]) < 1e-6
+
+
 # 3-D grids of 96 points along z (compile-time FFT kernels for the z
 # pass): adjoint FFT and DEAPODIZATION stops with density and coil maps,
 # partial and full coil blocks, sector widths 8 and 16, an anisotropic grid
