@@ -247,6 +247,11 @@ struct gnufft_plan_s {
   // workgroups per CU), at most `chunk`
   uint32_t round_target = 1024;
   WorkList work{};
+  // the chunks of each sector merged in runs of up to kInterpRun (the
+  // round-based interp stages a sector once per run); empty when no sector
+  // has more than one chunk
+  Chunk* runs_d = nullptr;
+  WorkList work_runs{};
   int lane_per_cu = 0;  // gnufft_plan_set_workgroups_per_cu (0: default)
   uint32_t* sector_chunks_d = nullptr;  // [S+1] first chunk of each sector
   uint2* reduce_d = nullptr;            // (first chunk, count) of multi-chunk sectors
@@ -307,12 +312,15 @@ void free_traj(gnufft_plan p) {
   dfree(p->sdc_d);
   dfree(p->centers_d);
   dfree(p->chunks_d);
+  dfree(p->runs_d);
   dfree(p->sector_chunks_d);
   dfree(p->reduce_d);
   dfree(p->scratch_d);
   p->traj_sorted_d = p->dens_sorted_d = nullptr;
   p->idx_d = p->assigned_d = p->sdc_d = p->centers_d = nullptr;
   p->chunks_d = nullptr;
+  p->runs_d = nullptr;
+  p->work_runs = WorkList{};
   p->sector_chunks_d = nullptr;
   p->reduce_d = nullptr;
   p->n_reduce = 0;
@@ -632,6 +640,19 @@ int build_chunks(gnufft_plan p, hipStream_t s) {
   if (!tasks.empty()) {
     int rc = copy_in((void**)&p->reduce_d, tasks.data(), tasks.size() * sizeof(uint2), GNUFFT_HOST, s);
     if (rc) return rc;
+    std::vector<Chunk> runs;
+    std::vector<uint32_t> run_counts;
+    for (uint32_t i = 0; i < total;) {
+      uint32_t j = i + 1;
+      while (j < total && j - i < (uint32_t)kInterpRun && h[j].sector == h[i].sector) j++;
+      runs.push_back(Chunk{h[i].sector, h[i].begin, h[j - 1].end, 0});
+      run_counts.push_back(h[j - 1].end - h[i].begin);
+      i = j;
+    }
+    rc = copy_in((void**)&p->runs_d, runs.data(), runs.size() * sizeof(Chunk), GNUFFT_HOST, s);
+    if (rc) return rc;
+    p->work_runs.chunks = p->runs_d;
+    partition_work(run_counts.data(), (uint32_t)runs.size(), &p->work_runs);
     HIP_TRY(hipStreamSynchronize(s));
   }
   return GNUFFT_OK;
@@ -1221,6 +1242,7 @@ int gnufft_interp_sorted(gnufft_plan p, const void* grid, void* data_sorted, uin
   a.grid = (const float2*)grid;
   a.crds = p->traj_sorted_d;
   a.work = p->work;
+  a.work_runs = p->work_runs;
   a.lane_per_cu = p->lane_per_cu;
   a.lut = p->lut_d;
   a.out = (float2*)data_sorted;
@@ -1314,6 +1336,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.grid = p->img_tmp_d;
     a.crds = p->traj_sorted_d;
     a.work = p->work;
+    a.work_runs = p->work_runs;
     a.lane_per_cu = p->lane_per_cu;
     a.lut = p->lut_d;
     a.out = (float2*)out;
@@ -1349,6 +1372,7 @@ int gnufft_forward(gnufft_plan p, const void* img, void* out, uint32_t n_coils, 
     a.grid = p->grid_d;
     a.crds = p->traj_sorted_d;
     a.work = p->work;
+    a.work_runs = p->work_runs;
     a.lane_per_cu = p->lane_per_cu;
     a.lut = p->lut_d;
     a.out = (float2*)out + (size_t)c0 * p->M;

@@ -258,7 +258,14 @@ struct InterpArgs {
   int n_coils;
   bool fold_shift;         // read grid at FORWARD-fftshifted positions
   int lane_per_cu;         // as in SpreadArgs
+  // runs of up to kInterpRun consecutive chunks of one sector, for the
+  // round-based interp (it stages a sector once per run; n_chunks 0: none)
+  WorkList work_runs;
 };
+// Chunks per interp run: 4 stages the dense k-space centre's sectors fewer
+// times at the 128^3 koosh workload (interp 0.62 -> 0.60 ms); 8 and whole
+// sectors leave a long tail of busy workgroups (0.67 and 1.62 ms).
+constexpr int kInterpRun = 4;
 hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStream_t s);
 
 

@@ -1810,8 +1810,10 @@ hipError_t launch_interp(const Geom& g, const InterpArgs& a, int chunk, hipStrea
   if (a.work.n_chunks == 0 || a.n_coils == 0) return hipSuccess;
   if (g.kw > 7 || g.P > 255) return hipErrorInvalidValue;
   if (lane_interp_ok(g, chunk)) return g.dim == 3 ? interp_lane<3>(g, a, s) : interp_lane<2>(g, a, s);
-  if (g.dim == 3) return nx_of(g) == 4 ? interp_nx<3, 4>(g, a, chunk, s) : interp_nx<3, 8>(g, a, chunk, s);
-  return nx_of(g) == 4 ? interp_nx<2, 4>(g, a, chunk, s) : interp_nx<2, 8>(g, a, chunk, s);
+  InterpArgs b = a;
+  if (a.work_runs.n_chunks > 0) b.work = a.work_runs;
+  if (g.dim == 3) return nx_of(g) == 4 ? interp_nx<3, 4>(g, b, chunk, s) : interp_nx<3, 8>(g, b, chunk, s);
+  return nx_of(g) == 4 ? interp_nx<2, 4>(g, b, chunk, s) : interp_nx<2, 8>(g, b, chunk, s);
 }
 
 // ---------------------------------------------------------------------------
