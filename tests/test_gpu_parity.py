@@ -173,11 +173,16 @@ def test_half_grid_positions(oracle, torch_dev, capi, case, C, lookup):
     _gridding_case(oracle, torch_dev, capi, traj, img, osf, kw, sw, C, lookup, seed=C + 3)
 
 
-def test_heavy_sector_chunks(oracle, torch_dev, capi):
-    """Thousands of samples in one sector (split over many workgroups)."""
+@pytest.mark.parametrize("n_heavy", [6000, 9000])
+def test_heavy_sector_chunks(oracle, torch_dev, capi, n_heavy):
+    """Thousands of samples in one sector (split over many workgroups): the
+    central sector holds 6007 or 9010 samples, 6 or 9 chunks of 1024, so the
+    3-D interp's runs of up to four chunks end full and partial (4 + 2,
+    4 + 4 + 1)."""
     rng = np.random.default_rng(1)
-    traj = np.concatenate([rng.uniform(-0.01, 0.01, (3, 6000)),
+    traj = np.concatenate([rng.uniform(-0.01, 0.01, (3, n_heavy)),
                            rng.uniform(-0.5, 0.5, (3, 500))], axis=1).astype(np.float32)
+    print("case heavy", n_heavy, flush=True)
     _gridding_case(oracle, torch_dev, capi, traj, (16, 16, 16), 2.0, 3, 8, 8, 0, seed=2)
 
 
