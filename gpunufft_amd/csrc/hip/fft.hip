@@ -48,6 +48,7 @@ constexpr int kMaxStripe = 32;    // lines per tile of a y / z pass (256 B rows)
 constexpr int kMaxPix = 4;        // image pixels per thread in the x passes
 constexpr int kMaxLen = kMaxPix * kFftThreads;  // longest transform (2048)
 constexpr int kCoilLoads = 4;  // per-pixel coil loads in flight in the x passes
+constexpr int kTileLoads = 4;  // grid loads per thread in flight in the adjoint x pass
 
 // ---------------------------------------------------------------------------
 // Small DFTs with kernel exp(sign * 2 pi i / R)
@@ -550,13 +551,33 @@ __device__ __forceinline__ void fft_adjoint_last_body(const LastPass& p, int blo
   for (int c0 = 0; c0 < p.coils; c0 += p.Cg) {
     const int ng = min(p.Cg, p.coils - c0);
     __syncthreads();  // the previous group's reads of buf are done
-    for (int c = 0; c < ng; c++) {
-      const float2* src = p.in + (long long)(c0 + c) * cin + (long long)l0 * p.G0;
-#pragma unroll 4
-      for (int e = threadIdx.x; e < per_coil; e += kFftThreads) {
-        const int q = qdiv(e, p.m_g0);
-        const int x = e - q * p.G0;
-        buf[(c * nq + q) * p.LP + x] = src[e];
+    // the group's lines, all coils in one flat loop: kTileLoads loads per
+    // thread in flight before their LDS writes (a loop per coil leaves one
+    // or two loads per thread, and a memory round trip per coil)
+    {
+      const int total = ng * per_coil;
+      const uint32_t m_pc = qdiv_const_dev(per_coil);
+      const float2* src = p.in + (long long)c0 * cin + (long long)l0 * p.G0;
+      for (int e0 = threadIdx.x; e0 < total; e0 += kTileLoads * kFftThreads) {
+        float2 v[kTileLoads];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; u++) {
+          const int e = e0 + u * kFftThreads;
+          if (e < total) {
+            const int c = qdiv(e, m_pc);
+            v[u] = src[(long long)c * cin + (e - c * per_coil)];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kTileLoads; u++) {
+          const int e = e0 + u * kFftThreads;
+          if (e < total) {
+            const int c = qdiv(e, m_pc);
+            const int r = e - c * per_coil;
+            const int q = qdiv(r, p.m_g0);
+            buf[(c * nq + q) * p.LP + (r - q * p.G0)] = v[u];
+          }
+        }
       }
     }
     __syncthreads();
