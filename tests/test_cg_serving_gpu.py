@@ -8,7 +8,9 @@
   in several solves, with eager solves and synchronisations in between, and
   every replay is compared with the eager result (rel-L2 <= 1e-6).
 - Serving: every request's result in a concurrent mix equals its result
-  when served alone (rel-L2 <= 1e-6).
+  when served alone (rel-L2 <= 1e-6), on shrunken requests and on
+  full-size ones; at full size the solver's objective also falls with the
+  iterations (a property CG fixes, no reference needed at that size).
 """
 
 import numpy as np
@@ -176,3 +178,61 @@ def test_serving_mix_equals_alone(oracle, env):
         r.arrival = 0.0
         alone.run([r])
         assert oracle.rel_l2(m, r.result) < 1e-6, r.rid
+
+
+def _cg_objective(torch, capi, req, plan, x):
+    """CG's objective f(x) = 1/2 (|A x|^2 + alpha |x|^2) - Re <A^H y, x> of
+    the request's operator A (sensitivities, density), in float64 on the
+    host from the device products. CG from x = 0 minimises f over growing
+    Krylov spaces, so f(0) = 0 > f(x_1) >= f(x_k) in exact arithmetic."""
+    C, M = req.kdata.shape
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    b = torch.empty_like(x)
+    plan.adjoint(dev(req.kdata), b, C)
+    ax = torch.empty((C, M), dtype=torch.complex64, device="cuda")
+    plan.forward(x, ax, C)
+    xh = x.cpu().numpy().astype(np.complex128).ravel()
+    bh = b.cpu().numpy().astype(np.complex128).ravel()
+    axh = ax.cpu().numpy().astype(np.complex128).ravel()
+    return 0.5 * (np.vdot(axh, axh).real + req.alpha * np.vdot(xh, xh).real) - np.vdot(bh, xh).real
+
+
+def test_serving_mix_full_size(oracle, env):
+    """Full-size serving requests (2-D N 368 with 28 coils on a 736-point
+    grid, N 192 and 224 with 22 and 25 coils, 3-D 152^3 with 8 coils on a
+    304^3 grid; osf 2): every result of the concurrent mix equals the
+    request served alone (rel-L2 <= 1e-6), and the solver's objective falls
+    from 0 at x = 0 to f(x_1) and further to f(x_6) for the largest 2-D and
+    the 3-D request."""
+    torch, capi = env
+    from gpunufft_amd import serve
+
+    reqs = serve.serving_mix(n_requests=4, rate_hz=500.0, seed=1, iterations=6)
+    print("full-size mix", [(r.img_dims, r.coils) for r in reqs], flush=True)
+    srv = serve.Server(slots=3, keep_results=True)
+    assert srv.run(reqs)["requests"] == 4
+    mixed = [r.result for r in reqs]
+    alone = serve.Server(slots=1, keep_results=True)
+    for r, m in zip(reqs, mixed):
+        r.arrival = 0.0
+        alone.run([r])
+        print("alone", r.rid, flush=True)
+        assert oracle.rel_l2(m, r.result) < 1e-6, r.rid
+    for r in (reqs[0], reqs[3]):
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        p = capi.Plan(r.img_dims, r.kernel_width, r.sector_width, r.osf)
+        p.set_trajectory(dev(r.traj), dev(r.dens))
+        p.set_sens(dev(r.sens), r.coils)
+        cg = capi.CGSense(p, r.coils, r.alpha)
+        kd = dev(r.kdata)
+        f = {}
+        for k in (1, 6):
+            x = torch.empty((1,) + tuple(r.img_dims[::-1]), dtype=torch.complex64, device="cuda")
+            cg.solve(kd, x, k, True)
+            torch.cuda.synchronize()
+            if k == 6:
+                assert oracle.rel_l2(r.result, x.cpu().numpy()) < 1e-6, r.rid
+            f[k] = _cg_objective(torch, capi, r, p, x)
+        print("objective", r.rid, f, flush=True)
+        assert f[1] < 0.0 and f[6] < f[1], (r.rid, f)
+        cg.close()
