@@ -428,6 +428,38 @@ def _adjointness(torch_dev, pl, img_shape, C, M, use_sens, seed):
 
 
 @pytest.mark.parametrize("cfg", ["radial2d_256_mc16", "koosh3d_128_mc8"])
+def test_full_size_against_oracle(oracle, torch_dev, capi, cfg):
+    """Full workload sizes (radial 256^2 with 16 coils, koosh 128^3 with 8
+    coils, sensitivities and density) against the oracle operator, run with
+    OpenMP over up to 16 host threads (private grids per thread): adjoint
+    and forward within rel-L2 1e-5."""
+    import os
+
+    w = W.config(cfg)
+    torch = torch_dev
+    traj, img, C = w["traj"], w["img_dims"], w["coils"]
+    M = traj.shape[1]
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    print("config", cfg, "oracle threads", threads, flush=True)
+    sens = W.gaussian_sens(img, C, seed=w["seed"])
+    kd = W.complex_normal(np.random.default_rng(w["seed"]), (C, M))
+    pl = capi.Plan(img, w["kw"], w["sw"], w["osf"], 0)
+    pl.set_trajectory(dev(torch, traj), dev(torch, w["dens"]))
+    pl.set_sens(dev(torch, sens), C)
+    op = oracle.Operator(traj, img, w["kw"], w["sw"], w["osf"], dens=w["dens"], sens=sens,
+                         order="stable", threads=threads)
+    x_ref = op.adjoint(kd)
+    k_ref = op.forward(x_ref)
+    x = torch.empty(x_ref.shape, dtype=torch.complex64, device="cuda")
+    pl.adjoint(dev(torch, kd), x, C)
+    k = torch.empty((C, M), dtype=torch.complex64, device="cuda")
+    pl.forward(dev(torch, x_ref), k, C)
+    ea, ef = oracle.rel_l2(host(x), x_ref), oracle.rel_l2(host(k), k_ref)
+    print("rel-L2 adjoint %.3g forward %.3g" % (ea, ef), flush=True)
+    assert ea < 1e-5 and ef < 1e-5, (ea, ef)
+
+
+@pytest.mark.parametrize("cfg", ["radial2d_256_mc16", "koosh3d_128_mc8"])
 def test_full_size_properties(torch_dev, capi, cfg):
     """Full workload sizes (radial 256^2 with 16 coils, koosh 128^3 with 8
     coils): adjointness and linearity (size-independent checks)."""
