@@ -650,7 +650,10 @@ int build_chunks(gnufft_plan p, hipStream_t s) {
       i = j;
     }
     rc = copy_in((void**)&p->runs_d, runs.data(), runs.size() * sizeof(Chunk), GNUFFT_HOST, s);
-    if (rc) return rc;
+    if (rc) {
+      (void)hipStreamSynchronize(s);  // the task upload above reads `tasks`
+      return rc;
+    }
     p->work_runs.chunks = p->runs_d;
     partition_work(run_counts.data(), (uint32_t)runs.size(), &p->work_runs);
     HIP_TRY(hipStreamSynchronize(s));
