@@ -1200,6 +1200,38 @@ __device__ inline void interp_rows(const float2* sub, int P, int Pmax, int bx, i
   *accy = ay;
 }
 
+// interp_rows with T rows per axis known at compile time (no branch per
+// row, so the LDS reads of every row can be in flight together). Callers
+// use it when no lane of the wave has more than T taps on any axis; the
+// same operations in the same order as interp_rows with those trips.
+template <int DIM, int NX, int CB, int T>
+__device__ inline void interp_rows_fixed(const float2* sub, int P, int Pmax, int bx, int by, int bz,
+                                         const float* w, int c, float* accx, float* accy) {
+  float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+  for (int tz = 0; tz < (DIM == 3 ? T : 1); tz++) {
+    const int iz = DIM == 3 ? min(bz + tz, Pmax) : 0;
+    const float wz = DIM == 3 ? w[1 + 2 * NX + tz] : 1.0f;
+#pragma unroll
+    for (int ty = 0; ty < T; ty++) {
+      const int iy = min(by + ty, Pmax);
+      const float wzy = DIM == 3 ? wz * w[1 + NX + ty] : w[1 + NX + ty];
+      const float2* row = sub + ((iz * P + iy) * P + bx) * CB + c;
+      float2 v[T];
+#pragma unroll
+      for (int tx = 0; tx < T; tx++) v[tx] = row[tx * CB];
+#pragma unroll
+      for (int tx = 0; tx < T; tx++) {
+        const float val = wzy * w[1 + tx];
+        ax += v[tx].x * val;
+        ay += v[tx].y * val;
+      }
+    }
+  }
+  *accx = ax;
+  *accy = ay;
+}
+
 template <int DIM, int NX, int CB, int NT = kThreads>
 __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t ci, int coil_block,
                                    int R) {
@@ -1307,8 +1339,12 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
       const int nz = DIM == 3 ? wave_trip(packed, 2, g.kw) : 1;
       float accx = 0.0f, accy = 0.0f;
       // x taps: NX, or 3 when NX is 4 and no lane of the wave has a fourth
-      // (wave-uniform; one branch per sample, the rows unrolled)
-      if (NX == 4 && wave_trip(packed, 0, g.kw) <= 3)
+      // (wave-uniform; one branch per sample, the rows unrolled); with three
+      // taps on every axis the rows run without a branch per row
+      const int nxw = wave_trip(packed, 0, g.kw);
+      if (NX == 4 && nxw <= 3 && ny <= 3 && nz <= 3)
+        interp_rows_fixed<DIM, NX, CB, 3>(sub, P, Pmax, bx, by, bz, w, c, &accx, &accy);
+      else if (NX == 4 && nxw <= 3)
         interp_rows<DIM, NX, CB, (NX == 4 ? 3 : NX)>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
       else
         interp_rows<DIM, NX, CB, NX>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
