@@ -1339,10 +1339,11 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
       const int nz = DIM == 3 ? wave_trip(packed, 2, g.kw) : 1;
       float accx = 0.0f, accy = 0.0f;
       // x taps: NX, or 3 when NX is 4 and no lane of the wave has a fourth
-      // (wave-uniform; one branch per sample, the rows unrolled); with three
-      // taps on every axis the rows run without a branch per row
+      // (wave-uniform; one branch per sample, the rows unrolled); at kernel
+      // width 3 with three taps on every axis the rows run without a branch
+      // per row (narrower kernels keep their shorter trips)
       const int nxw = wave_trip(packed, 0, g.kw);
-      if (NX == 4 && nxw <= 3 && ny <= 3 && nz <= 3)
+      if (NX == 4 && g.kw == 3 && nxw <= 3 && ny <= 3 && nz <= 3)
         interp_rows_fixed<DIM, NX, CB, 3>(sub, P, Pmax, bx, by, bz, w, c, &accx, &accy);
       else if (NX == 4 && nxw <= 3)
         interp_rows<DIM, NX, CB, (NX == 4 ? 3 : NX)>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
