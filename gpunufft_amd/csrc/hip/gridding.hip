@@ -1342,6 +1342,11 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
       // (wave-uniform; one branch per sample, the rows unrolled); at kernel
       // width 3 with three taps on every axis the rows run without a branch
       // per row (narrower kernels keep their shorter trips)
+      // the store's density weight and destination, loaded before the taps
+      // so that their latency overlaps them
+      const uint32_t s = ch.begin + r0 + ls;
+      const float dn = a.dens ? a.dens[s] : 1.0f;
+      const uint32_t dst = a.scatter ? a.scatter[s] : s;
       const int nxw = wave_trip(packed, 0, g.kw);
       if (NX == 4 && g.kw == 3 && nxw <= 3 && ny <= 3 && nz <= 3)
         interp_rows_fixed<DIM, NX, CB, 3>(sub, P, Pmax, bx, by, bz, w, c, &accx, &accy);
@@ -1349,14 +1354,12 @@ __device__ inline void interp_body(const Geom& g, const InterpArgs& a, uint32_t 
         interp_rows<DIM, NX, CB, (NX == 4 ? 3 : NX)>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
       else
         interp_rows<DIM, NX, CB, NX>(sub, P, Pmax, bx, by, bz, ny, nz, w, c, &accx, &accy);
-      const uint32_t s = ch.begin + r0 + ls;
       float rx = accx * a.scale, ry = accy * a.scale;
       if (a.dens) {
-        const float sq = sqrtf(a.dens[s]);
+        const float sq = sqrtf(dn);
         rx = rx * sq;
         ry = ry * sq;
       }
-      const uint32_t dst = a.scatter ? a.scatter[s] : s;
       a.out[(size_t)cg * a.coil_stride + dst] = make_float2(rx, ry);
     }
   }
@@ -1530,19 +1533,21 @@ __device__ inline void interp_lane_body(const Geom& g, const InterpArgs& a, uint
       const int jo = r * NT + (tid & ~3) + ps * (4 / CB) + (tid & 3) / CB;
       if (!lo_live || !coil_ok) continue;
       const int b[3] = {(int)(bo & 255u), (int)((bo >> 8) & 255u), (int)((bo >> 16) & 255u)};
+      // the store's density weight and destination, loaded before the taps
+      const uint32_t s = ch.begin + (uint32_t)jo;
+      const float dn = a.dens ? a.dens[s] : 1.0f;
+      const uint32_t dst = a.scatter ? a.scatter[s] : s;
       float ax, ay;
       if (any_wide)
         lane_gather<DIM, NX, CB>(sub, P, b, wo, qc, &ax, &ay);
       else
         lane_gather<DIM, KW, CB>(sub, P, b, wo, qc, &ax, &ay);
-      const uint32_t s = ch.begin + (uint32_t)jo;
       float rx = ax * a.scale, ry = ay * a.scale;
       if (a.dens) {
-        const float sq = sqrtf(a.dens[s]);
+        const float sq = sqrtf(dn);
         rx = rx * sq;
         ry = ry * sq;
       }
-      const uint32_t dst = a.scatter ? a.scatter[s] : s;
       a.out[(size_t)(coil0 + qc) * a.coil_stride + dst] = make_float2(rx, ry);
     }
   }
