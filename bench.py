@@ -68,6 +68,11 @@ def parse():
                     help="write profiles/koosh3d_128_mc8_kernel_trace.md from the "
                          "kernel_stats.csv of a rocprofv3 --kernel-trace --stats run of "
                          "--profile-steps 3 (no GPU needed)")
+    ap.add_argument("--trace-workload", default="koosh3d_128_mc8",
+                    choices=["koosh3d_128_mc8", "uneven_batch_2d"],
+                    help="with --summarize-trace: the workload the trace was taken of "
+                         "(uneven_batch_2d: a run of --workload uneven_batch_2d, written to "
+                         "profiles/uneven_batch_2d_kernel_trace.md)")
     ap.add_argument("--sources-sha256", default=None,
                     help="with --summarize-counters: the kernel-source hash the profiled run "
                          "printed (default: the hash of the sources here)")
@@ -205,11 +210,14 @@ def summarize_counters(paths, sha=None):
     print(json.dumps({"traffic": out["hbm_bytes_per_launch"], "kernels": list(kernels)}))
 
 
-def summarize_trace(path, sha=None):
+def summarize_trace(path, sha=None, workload="koosh3d_128_mc8"):
     """Kernel time table of a --profile-steps 3 kernel trace (three steps of
-    the headline workload after one plan set-up)."""
+    the headline workload after one plan set-up), or of a run of
+    --workload uneven_batch_2d (workload="uneven_batch_2d")."""
     import csv
     rows = list(csv.DictReader(open(path)))
+    if workload == "uneven_batch_2d":
+        return _summarize_batch_trace(rows, path, sha)
     lines = ["# Kernel trace: koosh3d 128^3 osf 1.5 kw 3 sw 8, 16384x256 samples, 8 coils, "
              "adjoint + forward x3", "",
              "Command (one MI355X, from the repository root R):", "", "```",
@@ -235,10 +243,48 @@ def summarize_trace(path, sha=None):
         f.write("\n".join(lines) + "\n")
 
 
+def _summarize_batch_trace(rows, path, sha):
+    """Kernel table of a `--workload uneven_batch_2d` trace: the batched
+    launches (kernel names with "batch") and the plans' own launches of the
+    plain path, with the batched time per step."""
+    def short(name):
+        if "rocprim" in name:
+            return "rocprim (sort/scan of the sector precompute)"
+        s = name.replace("(anonymous namespace)::", "").split("(")[0]
+        return s.replace("void ", "").replace("gnufft::", "")
+    batch = [r for r in rows if "batch" in r["Name"] and "gnufft" in r["Name"]]
+    steps = max((int(r["Calls"]) for r in batch if "spread_batch" in r["Name"]), default=0)
+    per_step = sum(float(r["TotalDurationNs"]) for r in batch) / 1e3 / steps if steps else 0.0
+    lines = ["# Kernel trace: uneven 2-D batch, 64 operators (N 96-384, 4-16 coils, spirals of "
+             "8-48 x 1024-4096 samples, sens + density, osf 2, kw 3, autotuned sector widths)", "",
+             "Command (one MI355X, from the repository root R):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+             "-d $R/gpurun_out/ktb -o run -- python3 $R/bench.py --workload uneven_batch_2d",
+             "python3 bench.py --summarize-trace $R/gpurun_out/ktb/run_kernel_stats.csv "
+             "--trace-workload uneven_batch_2d", "```", "",
+             "Kernel sources SHA-256 %s (bench.py sources_sha256)." % (sha or sources_sha256()), "",
+             "The run tunes the sector widths, then times the 64 operators one after another "
+             "(each plan's own launches) and as one batch (one launch per stage for all plans), "
+             "warm-up and timed steps alike. Kernels with `batch` in their name are the batched "
+             "launches: %d steps, **%.1f us of batched kernel time per step** (adjoint + forward "
+             "of all 64 operators)." % (steps, per_step), "",
+             "| kernel | calls | total us | avg us | per batched step us | % |",
+             "|---|---|---|---|---|---|"]
+    for r in rows:
+        b = r in batch
+        lines.append("| `%s` | %s | %.1f | %.1f | %s | %.1f |" % (
+            short(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e3,
+            float(r["AverageNs"]) / 1e3,
+            "%.1f" % (float(r["TotalDurationNs"]) / 1e3 / steps) if b and steps else "-",
+            float(r["Percentage"])))
+    with open(os.path.join(ROOT, "profiles", "uneven_batch_2d_kernel_trace.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
     args = parse()
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace, args.sources_sha256)
+        summarize_trace(args.summarize_trace, args.sources_sha256, args.trace_workload)
         return
     if args.summarize_counters:
         summarize_counters(args.summarize_counters, args.sources_sha256)
